@@ -1,0 +1,166 @@
+"""Full PIC cycle throughput on one MI355X.
+
+    python bench.py [--steps K] [--warmup W] [--config prod|small] [--cpu-steps C]
+
+Workload (default, "prod"): 1024x512 cells, ds 0.5, dt 0.05, 200 particles
+per cell per species (2 x 104,857,600), mi/me 1836, Te/Ti 10, probe bias
+-10 kTe/e, Jacobi with FP64 residual to 1e-6 (at most 500 sweeps), FP32
+particles and fields, int64 fixed-point charge.  Synthetic seeded initial
+load; the first steps after init are warm-up.  One step is one pass of the
+cycle deposit -> solve -> field -> move -> cc over both species.
+
+Prints one JSON line: particle-steps/s over the timed steps (inputs resident
+in HBM), the roofline of the dominant kernel (HIP events on the run's
+stream), the full-cycle algorithmic-byte roofline, and a CPU baseline (the
+oracle's OpenMP FP64 build on a bounded sample of the same workload).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+
+CONFIGS = {
+    "prod": dict(ncx=1024, ncy=512, ppc=200, mi=1836.0, kti=0.1, phi_p=-10.0),
+    "small": dict(ncx=128, ncy=64, ppc=20, mi=100.0, kti=1.0, phi_p=-5.0),
+}
+
+
+def workload_name(cfg, p):
+    return (f"{cfg['ncx']}x{cfg['ncy']} cells, {cfg['ppc']} ppc/species "
+            f"(2x{p.per_row * p.ncy} particles), mi/me={cfg['mi']:g}, "
+            f"Te/Ti={1.0 / cfg['kti']:g}, phi_p={cfg['phi_p']:g} kTe/e, "
+            f"Jacobi FP64 residual {p.jacobi_tol:g} (max {p.jacobi_max_sweeps} sweeps)")
+
+
+def cpu_baseline(cfg, steps):
+    """The oracle's OpenMP FP64 cycle on the same workload, timed for a few
+    steps after its own init (the CPU checker, never the measured path)."""
+    import ctypes as C
+
+    from oracle import oracle as O
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+    p = O.make_params(**cfg)
+    sim = O.Sim(p, omp=True)
+    sim.set_solver(1, 1e-6, 1e-6, 0, 500, 50)
+    sim.init()
+    t = p.dt
+    n = 0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        n += sim.count(0) + sim.count(1)
+        sim.step(t)
+        t += p.dt
+    el = time.perf_counter() - t0
+    del sim
+    return {"value": n / el, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{steps} full cycles of the same workload after init "
+                      f"(FP64, OpenMP, {threads} threads), {el:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", default="prod", choices=sorted(CONFIGS))
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-timing", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: a single run of this path lives on one device "
+              "(multi-GPU sharding is a design note, not part of this build)", file=sys.stderr)
+        sys.exit(2)
+
+    import cupic_amd as cp
+    name, cus = cp.init_dev(0)
+    cfg = CONFIGS[a.config]
+    p = cp.params(**cfg, jacobi_mode=cp.JACOBI_RESIDUAL, jacobi_tol=1e-6,
+                  jacobi_tol_init=1e-6, jacobi_max_sweeps=500, jacobi_check_every=50)
+    print(f"# {name}, {cus} CUs; {workload_name(cfg, p)}", file=sys.stderr, flush=True)
+    run = cp.Run(p)
+    t0 = time.perf_counter()
+    run.init_sim()
+    run.synchronize()
+    print(f"# init {time.perf_counter() - t0:.2f} s", file=sys.stderr, flush=True)
+    t = p.dt
+    for _ in range(a.warmup):
+        run.step(t)
+        t += p.dt
+    run.synchronize()
+    print("# warm-up done", file=sys.stderr, flush=True)
+    n0 = run.number_of_particles(0) + run.number_of_particles(1)
+    sw0, _ = run.solver_totals()
+    if not a.no_timing:
+        run.timing(True)
+        run.timing_read()
+    run.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        run.step(t)
+        t += p.dt
+    run.synchronize()
+    wall = time.perf_counter() - t0
+    n1 = run.number_of_particles(0) + run.number_of_particles(1)
+    sw1, _ = run.solver_totals()
+    kern = run.timing_read() if not a.no_timing else {}
+    run.timing(False)
+    ps = 0.5 * (n0 + n1) * a.steps
+    value = ps / wall
+    ms_step = 1e3 * wall / a.steps
+    S = (sw1 - sw0) / a.steps
+    nodes = p.ncx * (p.ncy + 1)
+    npart = 0.5 * (n0 + n1)
+    cycle_bytes = 84.0 * npart + nodes * (32.0 + 12.0 * S)
+    tally = run.probe_tally()
+    out = {
+        "metric": "particle-steps/s (full PIC cycle)",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": workload_name(cfg, p), "grid": [p.ncx, p.ncy],
+                   "ppc": cfg["ppc"], "particles": int(n1), "jacobi_mode": "fp64-residual",
+                   "jacobi_sweeps_per_step": S, "device": name},
+    }
+    if kern:
+        per = {k: (v[0] / v[1] if v[1] else 0.0) for k, v in kern.items()}
+        out["kernels_ms_per_launch"] = per
+        dom = "scatter" if per.get("scatter", 0) >= per.get("push_rank", 0) else "push_rank"
+        bytes_per_particle = 32.0 if dom == "scatter" else 16.0
+        dur = per[dom] * 1e-3
+        ach = bytes_per_particle * npart / dur if dur > 0 else 0.0
+        out["roofline"] = {"bound": "hbm", "kernel": "k_" + dom, "achieved": ach / 1e9,
+                           "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": ach / HBM_PEAK,
+                           "traffic": None,
+                           "bytes_per_particle": bytes_per_particle,
+                           "avg_launch_ms": per[dom]}
+    out["cycle_roofline"] = {"bytes_per_step": cycle_bytes,
+                             "achieved_GBps": cycle_bytes / (wall / a.steps) / 1e9,
+                             "frac": cycle_bytes / (wall / a.steps) / HBM_PEAK,
+                             "formula": "84*(Ne+Ni) + nodes*(32 + 12*S)"}
+    out["probe"] = {"total_absorbed": [int(v) for v in tally["total_probe"]],
+                    "charge_collected": tally["charge"]}
+    run.close()
+    if not a.no_cpu and a.cpu_steps > 0:
+        print("# cpu baseline ...", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(cfg, a.cpu_steps)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

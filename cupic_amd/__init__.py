@@ -1,0 +1,348 @@
+"""cupic_amd -- MI355X-native electrostatic PIC cycle (cupic's hot path).
+
+Python mirror of the reference's module entry points over the C-ABI in
+include/cupic.h (libcupic.so, hand-written HIP for gfx950).  Names follow
+the reference: ``init_sim``, ``charge_deposition``, ``poisson_solver``,
+``field_solver``, ``particle_mover``, ``cc``, ``number_of_particles``
+(src/init.h, src/mesh.h, src/particles.h, src/cc.h, src/diagnostic.h).
+Errors raise :class:`CupicError` instead of exiting the process.
+
+There is no CPU fallback: importing this package without the built HIP
+library raises ImportError.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcupic.so")
+
+CUPIC_OK = 0
+ERRORS = {-1: "ARG", -2: "HIP", -3: "ALLOC", -4: "CAPACITY", -5: "STATE", -6: "NUMERIC",
+          -7: "IO"}
+
+JACOBI_REFERENCE, JACOBI_RESIDUAL, JACOBI_REFINE = 0, 1, 2
+QUIRK_Q7_REFERENCE_KICK = 1
+MESH_RHO, MESH_PHI, MESH_EX, MESH_EY, MESH_CHARGE = 0, 1, 2, 3, 4
+ELECTRONS, IONS = 0, 1
+
+
+class CupicError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"cupic error {code} ({ERRORS.get(code, '?')}): {msg}")
+        self.code = code
+
+
+class Params(C.Structure):
+    """Mirror of cupic_params (include/cupic.h)."""
+    _fields_ = [
+        ("ncx", C.c_int32), ("ncy", C.c_int32),
+        ("ds", C.c_double), ("dt", C.c_double), ("n", C.c_double),
+        ("mi", C.c_double), ("kti", C.c_double), ("me", C.c_double), ("kte", C.c_double),
+        ("phi_p", C.c_double), ("eps0", C.c_double),
+        ("seed", C.c_uint64), ("run_id", C.c_uint64),
+        ("quirks", C.c_uint32),
+        ("jacobi_mode", C.c_int32), ("jacobi_min_sweeps", C.c_int32),
+        ("jacobi_max_sweeps", C.c_int32), ("jacobi_check_every", C.c_int32),
+        ("pad0", C.c_int32),
+        ("jacobi_tol", C.c_double), ("jacobi_tol_init", C.c_double),
+        ("Lx", C.c_double), ("Ly", C.c_double),
+        ("dtin_e", C.c_double), ("dtin_i", C.c_double),
+        ("per_row", C.c_int64),
+    ]
+
+    def copy(self):
+        p = Params()
+        C.pointer(p)[0] = self
+        return p
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built; run `python cupic_amd/build.py` "
+                          "(there is no CPU fallback)")
+    lib = C.CDLL(LIB_PATH)
+    P = C.POINTER(Params)
+    vp = C.c_void_p
+    i64p = C.POINTER(C.c_int64)
+    i32p = C.POINTER(C.c_int32)
+    dp = C.POINTER(C.c_double)
+    sig = {
+        "cupic_init_dev": [C.c_int, C.c_char_p, C.c_int, C.POINTER(C.c_int)],
+        "cupic_params_default": [P],
+        "cupic_params_derive": [P],
+        "cupic_params_from_input_file": [C.c_char_p, P, i32p, i32p, i32p, i32p],
+        "cupic_run_footprint": [P, C.c_int64, i64p],
+        "cupic_run_create": [P, C.c_int64, C.POINTER(vp)],
+        "cupic_run_destroy": [vp],
+        "cupic_synchronize": [vp],
+        "cupic_init_sim": [vp, dp],
+        "cupic_create_particles": [vp],
+        "cupic_initialize_mesh": [vp],
+        "cupic_adjust_leap_frog": [vp],
+        "cupic_charge_deposition": [vp],
+        "cupic_poisson_solver": [vp, C.c_double, i32p, dp],
+        "cupic_field_solver": [vp],
+        "cupic_particle_mover": [vp],
+        "cupic_cc": [vp, C.c_double],
+        "cupic_step": [vp, C.c_double],
+        "cupic_number_of_particles": [vp, C.c_int, i64p],
+        "cupic_probe_tally": [vp, i64p, i64p, i64p, i64p, dp],
+        "cupic_last_solve": [vp, i32p, dp],
+        "cupic_solver_totals": [vp, i64p, i64p],
+        "cupic_get_particles": [vp, C.c_int, vp, vp, vp, vp, C.c_int64, i64p],
+        "cupic_set_particles": [vp, C.c_int, vp, vp, vp, vp, C.c_int64],
+        "cupic_get_offsets": [vp, C.c_int, vp],
+        "cupic_get_mesh": [vp, C.c_int, vp, C.c_int64],
+        "cupic_set_phi": [vp, vp, C.c_int64],
+        "cupic_get_clock": [vp, dp, C.POINTER(C.c_uint64), i32p],
+        "cupic_set_clock": [vp, dp, C.POINTER(C.c_uint64), C.c_int32],
+        "cupic_particles_snapshot": [vp, C.c_int, C.c_char_p, C.c_double],
+        "cupic_mesh_snapshot": [vp, C.c_int, C.c_char_p],
+        "cupic_batch_create": [C.POINTER(vp), C.c_int32, C.c_int32, C.POINTER(vp)],
+        "cupic_batch_step": [vp, C.c_int32],
+        "cupic_batch_destroy": [vp],
+        "cupic_timing_enable": [vp, C.c_int32],
+        "cupic_timing_read": [vp, dp, i64p, C.c_int32],
+    }
+    for name, args in sig.items():
+        try:
+            f = getattr(lib, name)
+        except AttributeError as e:
+            raise ImportError(f"{LIB_PATH} lacks {name}: rebuild with "
+                              "`python cupic_amd/build.py`") from e
+        f.argtypes = args
+        f.restype = C.c_int
+    lib.cupic_last_error.argtypes = [vp]
+    lib.cupic_last_error.restype = C.c_char_p
+    lib.cupic_batch_last_error.argtypes = [vp]
+    lib.cupic_batch_last_error.restype = C.c_char_p
+    lib.cupic_timing_names.argtypes = []
+    lib.cupic_timing_names.restype = C.c_char_p
+    return lib
+
+
+lib = _load()
+
+
+def _check(code, handle=None, batch=False):
+    if code != CUPIC_OK:
+        fn = lib.cupic_batch_last_error if batch else lib.cupic_last_error
+        msg = fn(handle).decode(errors="replace")
+        raise CupicError(code, msg)
+
+
+def init_dev(device=0):
+    """init_dev (src/init.cu:15-55): select a device; returns (name, CUs)."""
+    buf = C.create_string_buffer(256)
+    cu = C.c_int(0)
+    _check(lib.cupic_init_dev(device, buf, 256, C.byref(cu)))
+    return buf.value.decode(), cu.value
+
+
+def params(**kw):
+    """Config-1 defaults, overridden by keyword (``ppc`` sets n = ppc/ds^3)."""
+    p = Params()
+    _check(lib.cupic_params_default(C.byref(p)))
+    ppc = kw.pop("ppc", None)
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise KeyError(k)
+        setattr(p, k, v)
+    if ppc is not None:
+        p.n = ppc / (p.ds * p.ds * p.ds)
+    p.eps0 = kw.get("eps0", 0.0)
+    _check(lib.cupic_params_derive(C.byref(p)))
+    return p
+
+
+def params_from_input_file(path):
+    """Reads the reference's input/input_data (src/init.cu:424-789)."""
+    p = Params()
+    v = [C.c_int32() for _ in range(4)]
+    _check(lib.cupic_params_from_input_file(path.encode(), C.byref(p), *[C.byref(x) for x in v]))
+    return p, dict(n_ini=v[0].value, n_prev=v[1].value, n_save=v[2].value, n_fin=v[3].value)
+
+
+def footprint(p, capacity=0):
+    b = C.c_int64()
+    _check(lib.cupic_run_footprint(C.byref(p), capacity, C.byref(b)))
+    return b.value
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+class Run:
+    """One simulation: owns its device buffers and stream (src/main.cu:42-44)."""
+
+    def __init__(self, p, capacity=0):
+        self.p = p.copy()
+        self.h = C.c_void_p()
+        _check(lib.cupic_run_create(C.byref(self.p), capacity, C.byref(self.h)))
+        self.nnx = self.p.ncx + 1
+        self.nny = self.p.ncy + 1
+
+    def close(self):
+        if self.h:
+            lib.cupic_run_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _c(self, code):
+        _check(code, self.h)
+
+    # init (src/init.h:32-38)
+    def init_sim(self):
+        t = C.c_double()
+        self._c(lib.cupic_init_sim(self.h, C.byref(t)))
+        return t.value
+
+    def create_particles(self):
+        self._c(lib.cupic_create_particles(self.h))
+
+    def initialize_mesh(self):
+        self._c(lib.cupic_initialize_mesh(self.h))
+
+    def adjust_leap_frog(self):
+        self._c(lib.cupic_adjust_leap_frog(self.h))
+
+    # mesh (src/mesh.h:27-29)
+    def charge_deposition(self):
+        self._c(lib.cupic_charge_deposition(self.h))
+
+    def poisson_solver(self, tol=-1.0):
+        s = C.c_int32()
+        r = C.c_double()
+        self._c(lib.cupic_poisson_solver(self.h, tol, C.byref(s), C.byref(r)))
+        return s.value, r.value
+
+    def field_solver(self):
+        self._c(lib.cupic_field_solver(self.h))
+
+    # particles (src/particles.h:28) and cc (src/cc.h:29)
+    def particle_mover(self):
+        self._c(lib.cupic_particle_mover(self.h))
+
+    def cc(self, t):
+        self._c(lib.cupic_cc(self.h, t))
+
+    def step(self, t):
+        self._c(lib.cupic_step(self.h, t))
+
+    def synchronize(self):
+        self._c(lib.cupic_synchronize(self.h))
+
+    # diagnostics (src/diagnostic.h:27-31)
+    def number_of_particles(self, species):
+        n = C.c_int64()
+        self._c(lib.cupic_number_of_particles(self.h, species, C.byref(n)))
+        return n.value
+
+    def probe_tally(self):
+        a = [(C.c_int64 * 2)() for _ in range(4)]
+        q = C.c_double()
+        self._c(lib.cupic_probe_tally(self.h, *a, C.byref(q)))
+        last_probe, last_top, last_inj, tot = [np.array(list(x), np.int64) for x in a]
+        return dict(probe=last_probe, top=last_top, injected=last_inj, total_probe=tot,
+                    charge=q.value)
+
+    def last_solve(self):
+        s = C.c_int32()
+        r = C.c_double()
+        self._c(lib.cupic_last_solve(self.h, C.byref(s), C.byref(r)))
+        return s.value, r.value
+
+    def solver_totals(self):
+        a, b = C.c_int64(), C.c_int64()
+        self._c(lib.cupic_solver_totals(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def particles(self, species):
+        n = self.number_of_particles(species)
+        a = [np.empty(n, np.float32) for _ in range(4)]
+        m = C.c_int64()
+        self._c(lib.cupic_get_particles(self.h, species, *[x.ctypes.data for x in a], n,
+                                        C.byref(m)))
+        return a
+
+    def set_particles(self, species, x, y, vx, vy):
+        a = [_f32(v) for v in (x, y, vx, vy)]
+        self._c(lib.cupic_set_particles(self.h, species, *[v.ctypes.data for v in a], len(a[0])))
+
+    def offsets(self, species):
+        o = np.empty(self.p.ncx * self.p.ncy + 1, np.int32)
+        self._c(lib.cupic_get_offsets(self.h, species, o.ctypes.data))
+        return o
+
+    def mesh(self, which):
+        """A mesh quantity in the reference layout, shape (nny, nnx)."""
+        dt = {MESH_RHO: np.float32, MESH_PHI: np.float64, MESH_EX: np.float32,
+              MESH_EY: np.float32, MESH_CHARGE: np.int64}[which]
+        a = np.empty(self.nnx * self.nny, dt)
+        self._c(lib.cupic_get_mesh(self.h, which, a.ctypes.data, a.size))
+        return a.reshape(self.nny, self.nnx)
+
+    def set_phi(self, phi):
+        a = np.ascontiguousarray(phi, np.float64).ravel()
+        self._c(lib.cupic_set_phi(self.h, a.ctypes.data, a.size))
+
+    def clock(self):
+        tin = (C.c_double * 2)()
+        ninj = (C.c_uint64 * 2)()
+        st = C.c_int32()
+        self._c(lib.cupic_get_clock(self.h, tin, ninj, C.byref(st)))
+        return list(tin), list(ninj), st.value
+
+    def set_clock(self, tin, ninj, started=1):
+        t = (C.c_double * 2)(*tin)
+        n = (C.c_uint64 * 2)(*ninj)
+        self._c(lib.cupic_set_clock(self.h, t, n, started))
+
+    def particles_snapshot(self, species, path, t):
+        self._c(lib.cupic_particles_snapshot(self.h, species, path.encode(), t))
+
+    def mesh_snapshot(self, which, path):
+        self._c(lib.cupic_mesh_snapshot(self.h, which, path.encode()))
+
+    def timing(self, on=True):
+        self._c(lib.cupic_timing_enable(self.h, 1 if on else 0))
+
+    def timing_read(self):
+        names = lib.cupic_timing_names().decode().split(",")
+        ms = (C.c_double * len(names))()
+        n = (C.c_int64 * len(names))()
+        self._c(lib.cupic_timing_read(self.h, ms, n, len(names)))
+        return {k: (ms[i], n[i]) for i, k in enumerate(names)}
+
+
+class Batch:
+    """Runs advanced together: one launch per kernel per step, optionally one
+    HIP graph per step."""
+
+    def __init__(self, runs, graph=True):
+        self.runs = list(runs)
+        arr = (C.c_void_p * len(self.runs))(*[r.h.value for r in self.runs])
+        self.h = C.c_void_p()
+        _check(lib.cupic_batch_create(arr, len(self.runs), 1 if graph else 0, C.byref(self.h)))
+
+    def step(self, n=1):
+        _check(lib.cupic_batch_step(self.h, n), self.h, batch=True)
+
+    def close(self):
+        if self.h:
+            lib.cupic_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,1168 @@
+// cupic_api.cpp -- host runtime behind the C-ABI (include/cupic.h).
+//
+// Owns each run's device buffers, sequences the kernels of the PIC cycle on
+// the run's stream, batches runs behind one set of launches (optionally one
+// HIP graph per step) and keeps the error convention of returning status
+// codes instead of exiting (the reference prints and exits,
+// src/cuda.cu:15-54).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "cupic.h"
+#include "cupic_launch.h"
+
+using namespace cupic;
+
+namespace {
+
+thread_local std::string g_err;
+
+enum TimeClass : int {
+    kTDeposit = 0,   // charge conversion (+ standalone deposit)
+    kTSolve,         // whole Poisson solve
+    kTField,
+    kTPushRank,      // pass 1
+    kTInject,        // clock + injection + far ranks
+    kTScan,
+    kTScatter,       // pass 2 (+ fused deposition)
+    kTFinish,        // far scatter + finish
+    kTClasses
+};
+const char *kTimeNames = "deposit,solve,field,push_rank,inject,scan,scatter,finish";
+
+int set_err(std::string *dst, int code, const std::string &msg)
+{
+    if (dst) *dst = msg;
+    g_err = msg;
+    return code;
+}
+
+}  // namespace
+
+#define HIPTRY(dst, expr)                                                               \
+    do {                                                                                \
+        hipError_t e_ = (expr);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            return set_err(dst, CUPIC_ERR_HIP,                                          \
+                           std::string(#expr) + ": " + hipGetErrorString(e_));          \
+    } while (0)
+
+struct cupic_run {
+    cupic_params p{};
+    RunDev h{};
+    RunDev *d_rd = nullptr;
+    int *d_plan = nullptr;
+    int nblk[kShapeCount] = {};
+    hipStream_t stream = nullptr;
+    std::vector<void *> allocs;
+    std::string err;
+    int moved = 0;          // particle_mover done, cc pending
+    bool acc_valid = false; // acc holds the charge of the current particles
+    bool initialised = false;
+    double t_next = 0.0;
+    int64_t footprint = 0;
+    bool timing = false;
+    struct Ev { int cls; hipEvent_t a, b; };
+    std::vector<Ev> evs;
+    std::vector<hipEvent_t> pool;
+    double t_ms[kTClasses] = {};
+    int64_t t_n[kTClasses] = {};
+};
+
+struct cupic_batch {
+    std::vector<cupic_run *> runs;
+    RunDev *d_rds = nullptr;
+    int *d_plan = nullptr;   // kShapeCount * (nruns + 1)
+    int nblk[kShapeCount] = {};
+    hipStream_t stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool use_graph = false;
+    int mode = 0, check = 50, max_sweeps = 0;
+    std::string err;
+};
+
+namespace cupic {
+
+int shape_blocks(const RunDev &h, Shape s)
+{
+    switch (s) {
+    case kShapeNodes: return (h.nnodes + kBlock - 1) / kBlock;
+    case kShapeTasks: return (2 * h.ncy * h.tpr + 3) / 4;
+    case kShapeCellBlk: return 2 * ((h.ncells + kScanCells - 1) / kScanCells);
+    case kShapeRunSp: return 2;
+    case kShapeRun: return 1;
+    case kShapeFar: return 2 * ((h.far_cap + kBlock - 1) / kBlock);
+    case kShapeInject: return 2 * ((h.max_inject + kBlock - 1) / kBlock);
+    case kShapeRows: return 2 * h.ncy;
+    case kShapeInit: return 2 * ((h.n_init + kBlock - 1) / kBlock);
+    default: return 0;
+    }
+}
+
+}  // namespace cupic
+
+// ---------------------------------------------------------------- kernel sequences
+
+namespace {
+
+// A sequence target: a single run or a batch.
+struct Target {
+    const RunDev *rds;
+    const int *plan;      // kShapeCount rows of (nruns + 1)
+    const int *nblk;
+    int nruns;
+    hipStream_t stream;
+    cupic_run *timed;     // non-null: record timing events for this run
+    Launch L(Shape s) const
+    {
+        return Launch{rds, plan + (size_t)s * (nruns + 1), nruns, nblk[s], stream};
+    }
+};
+
+Target target_of(cupic_run *r)
+{
+    return Target{r->d_rd, r->d_plan, r->nblk, 1, r->stream, r->timing ? r : nullptr};
+}
+
+struct TimeScope {
+    cupic_run *r;
+    int cls;
+    hipEvent_t a = nullptr, b = nullptr;
+    TimeScope(const Target &T, int c) : r(T.timed), cls(c)
+    {
+        if (!r) return;
+        a = take(); b = take();
+        (void)hipEventRecord(a, r->stream);
+    }
+    ~TimeScope()
+    {
+        if (!r) return;
+        (void)hipEventRecord(b, r->stream);
+        r->evs.push_back({cls, a, b});
+    }
+    hipEvent_t take()
+    {
+        if (!r->pool.empty()) { hipEvent_t e = r->pool.back(); r->pool.pop_back(); return e; }
+        hipEvent_t e;
+        (void)hipEventCreate(&e);
+        return e;
+    }
+};
+
+void seq_solve(const Target &T, int use_args, int mode, double tol, int min_sweeps,
+               int max_sweeps, int check)
+{
+    TimeScope ts(T, kTSolve);
+    launch_solver_begin(T.L(kShapeRun), use_args, mode, tol, min_sweeps, max_sweeps, check);
+    Launch N = T.L(kShapeNodes);
+    if (mode == 0) {
+        for (int s = 0; s < max_sweeps; s++) launch_jacobi_sweep(N, s, 0, -1, 0);
+        launch_solver_end(T.L(kShapeRun), 0);
+        return;
+    }
+    int c = 0, s = 0;
+    if (check < 1) check = 1;
+    for (;;) {
+        if (mode == 2 && c > 0) launch_ir_update(N, s);
+        launch_residual(N, c, s, mode);
+        if (s >= max_sweeps) break;
+        int k = std::min(check, max_sweeps - s);
+        for (int q = 0; q < k; q++)
+            launch_jacobi_sweep(N, s + q, mode, q == 0 ? c : -1, (mode == 2 && q == 0) ? 1 : 0);
+        s += k;
+        c += 1;
+    }
+    launch_solver_end(T.L(kShapeRun), c);
+}
+
+void seq_rho(const Target &T)
+{
+    TimeScope ts(T, kTDeposit);
+    launch_rho_convert(T.L(kShapeNodes));
+}
+
+void seq_field(const Target &T)
+{
+    TimeScope ts(T, kTField);
+    launch_field(T.L(kShapeNodes));
+}
+
+void seq_mover(const Target &T)
+{
+    TimeScope ts(T, kTPushRank);
+    launch_push_rank(T.L(kShapeTasks));
+}
+
+void seq_cc(const Target &T, int use_host_t, double t, int advance_t)
+{
+    {
+        TimeScope ts(T, kTInject);
+        launch_clock(T.L(kShapeRun), use_host_t, t);
+        launch_inject(T.L(kShapeInject));
+        launch_far_rank(T.L(kShapeFar));
+    }
+    {
+        TimeScope ts(T, kTScan);
+        launch_scan_a(T.L(kShapeCellBlk));
+        launch_scan_b(T.L(kShapeRunSp));
+        launch_scan_c(T.L(kShapeCellBlk));
+    }
+    {
+        TimeScope ts(T, kTScatter);
+        launch_scatter(T.L(kShapeTasks));
+    }
+    {
+        TimeScope ts(T, kTFinish);
+        launch_far_scatter(T.L(kShapeFar));
+        launch_finish(T.L(kShapeRun), advance_t);
+    }
+}
+
+template <class T>
+T *dev_alloc(cupic_run *r, size_t count, int *status)
+{
+    void *p = nullptr;
+    size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+        *status = CUPIC_ERR_ALLOC;
+        return nullptr;
+    }
+    (void)hipMemset(p, 0, bytes);
+    r->allocs.push_back(p);
+    r->footprint += (int64_t)bytes;
+    return (T *)p;
+}
+
+int check_launch(cupic_run *r, const char *what)
+{
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+        return set_err(&r->err, CUPIC_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+    return CUPIC_OK;
+}
+
+int read_state(cupic_run *r, RunState *out)
+{
+    HIPTRY(&r->err, hipMemcpyAsync(out, r->h.st, sizeof(RunState), hipMemcpyDeviceToHost,
+                                   r->stream));
+    HIPTRY(&r->err, hipStreamSynchronize(r->stream));
+    if (out->err) {
+        std::string m = "device error flags:";
+        if (out->err & kErrFarOverflow) m += " far-list overflow;";
+        if (out->err & kErrCapacity) m += " particle capacity exceeded;";
+        if (out->err & kErrInjectCap) m += " injection count above max_inject;";
+        if (out->err & kErrSortInvariant) m += " sort invariant broken;";
+        return set_err(&r->err, CUPIC_ERR_CAPACITY, m);
+    }
+    return CUPIC_OK;
+}
+
+int write_state(cupic_run *r, const RunState &s)
+{
+    HIPTRY(&r->err, hipMemcpyAsync(r->h.st, &s, sizeof(RunState), hipMemcpyHostToDevice,
+                                   r->stream));
+    HIPTRY(&r->err, hipStreamSynchronize(r->stream));
+    return CUPIC_OK;
+}
+
+// host copy of the device cell_of (same float operations, no contraction)
+inline int host_cell(const RunDev &R, float x, float y)
+{
+    float tx = x * R.inv_ds;
+    float ty = y * R.inv_ds;
+    float bx = std::fmin(std::fmax(std::floor(tx), 0.0f), (float)(R.ncx - 1));
+    float by = std::fmin(std::fmax(std::floor(ty), 0.0f), (float)(R.ncy - 1));
+    return (int)by * R.ncx + (int)bx;
+}
+
+int validate(const cupic_params *p, std::string *err)
+{
+    if (p->ncx < 4 || p->ncy < 2) return set_err(err, CUPIC_ERR_ARG, "need ncx >= 4 and ncy >= 2");
+    if (p->ncx > 4096) return set_err(err, CUPIC_ERR_ARG, "ncx above 4096 is not supported");
+    if (!(p->ds > 0) || !(p->dt > 0) || !(p->n > 0))
+        return set_err(err, CUPIC_ERR_ARG, "ds, dt and n must be positive");
+    if (!(p->mi > 0) || !(p->kti > 0))
+        return set_err(err, CUPIC_ERR_ARG, "mi and kti must be positive");
+    if (p->jacobi_mode < 0 || p->jacobi_mode > 2)
+        return set_err(err, CUPIC_ERR_ARG, "jacobi_mode must be 0, 1 or 2");
+    if (p->jacobi_max_sweeps < 1 || p->jacobi_max_sweeps > 1000000)
+        return set_err(err, CUPIC_ERR_ARG, "jacobi_max_sweeps out of range");
+    double ncells = (double)p->ncx * p->ncy;
+    if (ncells * 10.0 > 2.0e9) return set_err(err, CUPIC_ERR_ARG, "grid too large");
+    if ((double)p->per_row * p->ncy > 1.9e9)
+        return set_err(err, CUPIC_ERR_ARG, "more than 1.9e9 particles per species");
+    return CUPIC_OK;
+}
+
+struct Sizes {
+    int64_t cap, n_init, far_cap, max_inject, nnodes, ncells, nblk_scan;
+    int C;
+};
+
+Sizes sizes_of(const cupic_params *p, int64_t capacity_hint)
+{
+    Sizes z;
+    z.n_init = p->per_row * p->ncy;
+    double per_step = p->dt / std::min(p->dtin_e, p->dtin_i);
+    z.max_inject = (int64_t)std::ceil(per_step * 2.0) + 1024;
+    z.far_cap = std::max<int64_t>(65536, 4 * z.max_inject);
+    z.cap = capacity_hint > 0 ? capacity_hint : z.n_init + z.n_init / 8 + 4 * z.max_inject + 4096;
+    z.nnodes = (int64_t)p->ncx * (p->ncy + 1);
+    z.ncells = (int64_t)p->ncx * p->ncy;
+    z.nblk_scan = (z.ncells + kScanCells - 1) / kScanCells;
+    double ppc = (double)p->per_row / p->ncx;
+    int C = (int)std::lround(2048.0 / std::max(ppc, 1.0));
+    z.C = std::max(1, std::min(kCmax, C));
+    return z;
+}
+
+// [ncy+1][ncx] periodic -> reference layout [nny][nnx] with column nnx-1 = column 0
+template <class T>
+void to_reference(const std::vector<T> &in, int ncx, int ncy, T *out)
+{
+    int nnx = ncx + 1;
+    for (int j = 0; j <= ncy; j++) {
+        for (int i = 0; i < ncx; i++) out[(size_t)j * nnx + i] = in[(size_t)j * ncx + i];
+        out[(size_t)j * nnx + ncx] = in[(size_t)j * ncx];
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- C-ABI
+
+extern "C" {
+
+int cupic_init_dev(int device, char *name, int name_len, int *compute_units)
+{
+    int n = 0;
+    HIPTRY(nullptr, hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return set_err(nullptr, CUPIC_ERR_ARG, "no such device");
+    HIPTRY(nullptr, hipSetDevice(device));
+    hipDeviceProp_t prop;
+    HIPTRY(nullptr, hipGetDeviceProperties(&prop, device));
+    if (name && name_len > 0) {
+        std::snprintf(name, (size_t)name_len, "%s (%s)", prop.name, prop.gcnArchName);
+    }
+    if (compute_units) *compute_units = prop.multiProcessorCount;
+    return CUPIC_OK;
+}
+
+int cupic_params_default(cupic_params *p)
+{
+    if (!p) return set_err(nullptr, CUPIC_ERR_ARG, "null params");
+    std::memset(p, 0, sizeof(*p));
+    p->ncx = 128;
+    p->ncy = 64;
+    p->ds = 0.5;
+    p->dt = 0.05;
+    p->n = 20.0 / (0.5 * 0.5 * 0.5);
+    p->mi = 100.0;
+    p->kti = 1.0;
+    p->me = 1.0;
+    p->kte = 1.0;
+    p->phi_p = -5.0;
+    p->eps0 = 0.0;
+    p->seed = 20261203ull;
+    p->run_id = 0;
+    p->quirks = 0;
+    p->jacobi_mode = CUPIC_JACOBI_RESIDUAL;
+    p->jacobi_min_sweeps = 0;
+    p->jacobi_max_sweeps = 500;
+    p->jacobi_check_every = 50;
+    p->jacobi_tol = 1e-6;
+    p->jacobi_tol_init = 1e-6;
+    return cupic_params_derive(p);
+}
+
+// Normalised derived values, src/init.cu:451-789 (eps0 reduces to n).
+int cupic_params_derive(cupic_params *p)
+{
+    if (!p) return set_err(nullptr, CUPIC_ERR_ARG, "null params");
+    const double pi = 3.1415926535897932;  // src/stdh.h:26
+    if (p->me == 0.0) p->me = 1.0;
+    if (p->kte == 0.0) p->kte = 1.0;
+    if (p->eps0 == 0.0) p->eps0 = p->n;
+    if (p->jacobi_check_every <= 0) p->jacobi_check_every = 50;
+    p->Lx = p->ds * (double)p->ncx;
+    p->Ly = p->ds * (double)p->ncy;
+    p->per_row = (int64_t)(p->n * p->Lx * p->ds * p->ds);
+    p->dtin_i = std::sqrt(2.0 * pi * p->mi / p->kti) / (p->n * p->Lx * p->ds);
+    p->dtin_e = std::sqrt(2.0 * pi) / (p->n * p->Lx * p->ds);
+    return validate(p, nullptr);
+}
+
+// Positional reader of input/input_data (src/init.cu:424-447): a header
+// line, then 13 "key = value;" lines read by position (keys ignored).
+int cupic_params_from_input_file(const char *path, cupic_params *p, int32_t *n_ini,
+                                 int32_t *n_prev, int32_t *n_save, int32_t *n_fin)
+{
+    if (!path || !p) return set_err(nullptr, CUPIC_ERR_ARG, "null argument");
+    FILE *f = std::fopen(path, "r");
+    if (!f) return set_err(nullptr, CUPIC_ERR_IO, std::string("cannot open ") + path);
+    char line[256];
+    double v[14] = {};
+    int got = 0;
+    if (!std::fgets(line, sizeof line, f)) { std::fclose(f); return set_err(nullptr, CUPIC_ERR_IO, "empty input file"); }
+    for (int k = 1; k <= 13; k++) {
+        if (!std::fgets(line, sizeof line, f)) break;
+        double d;
+        if (std::sscanf(line, "%*s = %lf;", &d) != 1) break;
+        v[k] = d;
+        got++;
+    }
+    std::fclose(f);
+    if (got != 13) return set_err(nullptr, CUPIC_ERR_IO, "input file: expected 13 value lines");
+    const double CST_ME = 9.109e-31, CST_E = 1.602e-19, CST_KB = 1.381e-23,
+                 CST_EPSILON = 8.854e-12;  // src/init.h:24-27
+    double ne = v[5], Te = v[6], beta = v[7], gamma = v[8], phi_v = v[9];
+    cupic_params_default(p);
+    p->ncx = (int32_t)v[10];
+    p->ncy = (int32_t)v[11];
+    p->ds = v[12];
+    p->dt = v[13];
+    double Dl = std::sqrt(CST_EPSILON * CST_KB * Te / (ne * CST_E * CST_E));  // src/init.cu:729
+    p->n = ne * Dl * Dl * Dl;                                                // src/init.cu:551
+    p->mi = gamma;
+    p->kti = beta;
+    p->phi_p = phi_v * (CST_E / (CST_KB * Te));                              // src/init.cu:534
+    double eps = CST_EPSILON;                                                // src/init.cu:622-626
+    eps /= std::pow(Dl * std::sqrt(CST_ME / (CST_KB * Te)), 2);
+    eps /= CST_E * CST_E;
+    eps *= Dl * Dl * Dl;
+    eps *= CST_ME;
+    p->eps0 = eps;
+    p->jacobi_mode = CUPIC_JACOBI_REFERENCE;
+    p->jacobi_tol = 1.0e-4;       // src/main.cu:64
+    p->jacobi_tol_init = 1.0e-3;  // src/init.cu:254
+    p->jacobi_min_sweeps = std::max(p->ncx + 1, p->ncy + 1) + 1;
+    p->jacobi_max_sweeps = std::max(500, p->jacobi_min_sweeps + 1);
+    if (n_ini) *n_ini = (int32_t)v[1];
+    if (n_prev) *n_prev = (int32_t)v[2];
+    if (n_save) *n_save = (int32_t)v[3];
+    if (n_fin) *n_fin = (int32_t)v[4];
+    return cupic_params_derive(p);
+}
+
+int cupic_run_footprint(const cupic_params *pin, int64_t capacity_hint, int64_t *bytes)
+{
+    if (!pin || !bytes) return set_err(nullptr, CUPIC_ERR_ARG, "null argument");
+    cupic_params p = *pin;
+    int st = cupic_params_derive(&p);
+    if (st) return st;
+    Sizes z = sizes_of(&p, capacity_hint);
+    int64_t b = 0;
+    b += 2 * 2 * 4 * z.cap * 4;                  // particles
+    b += 2 * 2 * (z.ncells + 1) * 4;             // offsets
+    b += 2 * z.ncells * (9 + 10 + 1 + 1) * 4;    // cnt, base, far_cnt, tot
+    b += 2 * z.far_cap * (int64_t)sizeof(FarRec) + 2 * z.max_inject * 8;
+    b += z.nnodes * 60;                          // grids
+    b += (int64_t)(p.jacobi_max_sweeps + 2) * 12;
+    *bytes = b;
+    return CUPIC_OK;
+}
+
+int cupic_run_create(const cupic_params *pin, int64_t capacity_hint, cupic_run **out)
+{
+    if (!pin || !out) return set_err(nullptr, CUPIC_ERR_ARG, "null argument");
+    *out = nullptr;
+    cupic_params p = *pin;
+    int status = cupic_params_derive(&p);
+    if (status) return status;
+    Sizes z = sizes_of(&p, capacity_hint);
+    if (z.cap < z.n_init) return set_err(nullptr, CUPIC_ERR_ARG, "capacity below initial load");
+    if (z.cap > 2000000000LL) return set_err(nullptr, CUPIC_ERR_ARG, "capacity too large");
+    cupic_run *r = new cupic_run();
+    r->p = p;
+    RunDev &h = r->h;
+    h.ncx = p.ncx;
+    h.ncy = p.ncy;
+    h.ncells = (int)z.ncells;
+    h.nnodes = (int)z.nnodes;
+    h.C = z.C;
+    h.tpr = (p.ncx + z.C - 1) / z.C;
+    h.cap = (int)z.cap;
+    h.far_cap = (int)z.far_cap;
+    h.max_inject = (int)z.max_inject;
+    h.n_init = (int)z.n_init;
+    h.per_row = (int)p.per_row;
+    h.max_sweeps_alloc = p.jacobi_max_sweeps;
+    h.step_mode = p.jacobi_mode;
+    h.step_min = p.jacobi_min_sweeps;
+    h.step_max = p.jacobi_max_sweeps;
+    h.step_check = p.jacobi_check_every;
+    h.step_tol = p.jacobi_tol;
+    h.init_tol = p.jacobi_tol_init;
+    h.inv_ds = (float)(1.0 / p.ds);
+    h.Lx_f = (float)p.Lx;
+    h.Ly_f = (float)p.Ly;
+    h.dt_f = (float)p.dt;
+    h.ds = p.ds;
+    h.dt = p.dt;
+    h.Lx = p.Lx;
+    h.Ly = p.Ly;
+    h.eps0 = p.eps0;
+    h.phi_p = p.phi_p;
+    h.b_interior = 0x1p-32 / (p.ds * p.eps0);
+    h.rho_interior = 0x1p-32 / (p.ds * p.ds * p.ds);
+    h.rho_edge = 0x1p-32 / (0.5 * p.ds * p.ds * p.ds);
+    for (int s = 0; s < 2; s++) {
+        double q = s == 0 ? -1.0 : 1.0;
+        double m = s == 0 ? p.me : p.mi;
+        double kt = s == 0 ? p.kte : p.kti;
+        h.sp[s].q = q;
+        h.sp[s].m = m;
+        h.sp[s].sigma = std::sqrt(kt / m);
+        h.sp[s].dtin = s == 0 ? p.dtin_e : p.dtin_i;
+        h.sp[s].qdtm = (float)(q * p.dt / m);
+        h.sp[s].hkick = (float)(-0.5 * q * p.dt / m);
+    }
+    h.seed = p.seed;
+    h.run_id = p.run_id;
+    h.quirks = p.quirks;
+    int st = CUPIC_OK;
+    for (int s = 0; s < 2 && !st; s++) {
+        for (int b = 0; b < 2; b++) {
+            h.px[s][b] = dev_alloc<float>(r, z.cap, &st);
+            h.py[s][b] = dev_alloc<float>(r, z.cap, &st);
+            h.pvx[s][b] = dev_alloc<float>(r, z.cap, &st);
+            h.pvy[s][b] = dev_alloc<float>(r, z.cap, &st);
+            h.off[s][b] = dev_alloc<int>(r, z.ncells + 1, &st);
+        }
+        h.cnt[s] = dev_alloc<int>(r, z.ncells * 9, &st);
+        h.base[s] = dev_alloc<int>(r, z.ncells * 10, &st);
+        h.far_cnt[s] = dev_alloc<int>(r, z.ncells, &st);
+        h.far[s] = dev_alloc<FarRec>(r, z.far_cap, &st);
+        h.inj_tin[s] = dev_alloc<double>(r, z.max_inject, &st);
+        h.tot[s] = dev_alloc<int>(r, z.ncells, &st);
+        h.blk[s] = dev_alloc<int>(r, z.nblk_scan, &st);
+    }
+    h.acc = dev_alloc<unsigned long long>(r, z.nnodes, &st);
+    h.rho = dev_alloc<float>(r, z.nnodes, &st);
+    h.b32 = dev_alloc<float>(r, z.nnodes, &st);
+    h.b64 = dev_alloc<double>(r, z.nnodes, &st);
+    h.phi32[0] = dev_alloc<float>(r, z.nnodes, &st);
+    h.phi32[1] = dev_alloc<float>(r, z.nnodes, &st);
+    h.phi64 = dev_alloc<double>(r, z.nnodes, &st);
+    h.d32[0] = dev_alloc<float>(r, z.nnodes, &st);
+    h.d32[1] = dev_alloc<float>(r, z.nnodes, &st);
+    h.r32 = dev_alloc<float>(r, z.nnodes, &st);
+    h.E = dev_alloc<float2>(r, z.nnodes, &st);
+    h.sweep_max = dev_alloc<unsigned int>(r, (size_t)p.jacobi_max_sweeps + 1, &st);
+    h.resid = dev_alloc<unsigned long long>(r, (size_t)p.jacobi_max_sweeps + 2, &st);
+    h.st = dev_alloc<RunState>(r, 1, &st);
+    r->d_rd = dev_alloc<RunDev>(r, 1, &st);
+    r->d_plan = dev_alloc<int>(r, (size_t)kShapeCount * 2, &st);
+    if (st) {
+        cupic_run_destroy(r);
+        return set_err(nullptr, CUPIC_ERR_ALLOC, "device allocation failed");
+    }
+    std::vector<int> plan(kShapeCount * 2);
+    for (int s = 0; s < kShapeCount; s++) {
+        r->nblk[s] = shape_blocks(h, (Shape)s);
+        plan[2 * s] = 0;
+        plan[2 * s + 1] = r->nblk[s];
+    }
+    if (hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipMemcpy(r->d_rd, &h, sizeof(RunDev), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(r->d_plan, plan.data(), plan.size() * sizeof(int), hipMemcpyHostToDevice) !=
+            hipSuccess ||
+        hipDeviceSynchronize() != hipSuccess) {
+        cupic_run_destroy(r);
+        return set_err(nullptr, CUPIC_ERR_HIP, "run setup failed");
+    }
+    *out = r;
+    return CUPIC_OK;
+}
+
+int cupic_run_destroy(cupic_run *r)
+{
+    if (!r) return CUPIC_OK;
+    if (r->stream) (void)hipStreamSynchronize(r->stream);
+    for (auto &e : r->evs) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
+    for (auto e : r->pool) (void)hipEventDestroy(e);
+    for (void *p : r->allocs) (void)hipFree(p);
+    if (r->stream) (void)hipStreamDestroy(r->stream);
+    delete r;
+    return CUPIC_OK;
+}
+
+const char *cupic_last_error(const cupic_run *r)
+{
+    if (r && !r->err.empty()) return r->err.c_str();
+    return g_err.c_str();
+}
+
+int cupic_synchronize(cupic_run *r)
+{
+    if (!r) return set_err(nullptr, CUPIC_ERR_ARG, "null run");
+    HIPTRY(&r->err, hipStreamSynchronize(r->stream));
+    RunState s;
+    return read_state(r, &s);
+}
+
+// ---- init
+
+int cupic_create_particles(cupic_run *r)
+{
+    if (!r) return set_err(nullptr, CUPIC_ERR_ARG, "null run");
+    Target T = target_of(r);
+    launch_init_finish(T.L(kShapeRun));
+    launch_create(T.L(kShapeInit));
+    launch_init_sort(T.L(kShapeRows));
+    r->acc_valid = false;
+    r->moved = 0;
+    r->initialised = true;
+    r->t_next = r->p.dt;
+    return check_launch(r, "create_particles");
+}
+
+int cupic_charge_deposition(cupic_run *r)
+{
+    if (!r) return set_err(nullptr, CUPIC_ERR_ARG, "null run");
+    if (r->moved) return set_err(&r->err, CUPIC_ERR_STATE, "charge_deposition between mover and cc");
+    Target T = target_of(r);
+    if (!r->acc_valid) {
+        HIPTRY(&r->err, hipMemsetAsync(r->h.acc, 0, (size_t)r->h.nnodes * 8, r->stream));
+        TimeScope ts(T, kTDeposit);
+        launch_deposit(T.L(kShapeTasks));
+    }
+    seq_rho(T);
+    r->acc_valid = false;
+    return check_launch(r, "charge_deposition");
+}
+
+int cupic_initialize_mesh(cupic_run *r)
+{
+    if (!r) return set_err(nullptr, CUPIC_ERR_ARG, "null run");
+    Target T = target_of(r);
+    launch_phi_linear(T.L(kShapeNodes));
+    r->acc_valid = false;
+    int st = cupic_charge_deposition(r);
+    if (st) return st;
+    seq_solve(T, 1, r->p.jacobi_mode, r->p.jacobi_tol_init, r->p.jacobi_min_sweeps,
+              r->p.jacobi_max_sweeps, r->p.jacobi_check_every);
+    seq_field(T);
+    return check_launch(r, "initialize_mesh");
+}
+
+int cupic_adjust_leap_frog(cupic_run *r)
+{
+    if (!r) return set_err(nullptr, CUPIC_ERR_ARG, "null run");
+    launch_half_kick(target_of(r).L(kShapeInit));
+    return check_launch(r, "adjust_leap_frog");
+}
+
+int cupic_init_sim(cupic_run *r, double *t)
+{
+    int st = cupic_create_particles(r);
+    if (!st) st = cupic_initialize_mesh(r);
+    if (!st) st = cupic_adjust_leap_frog(r);
+    if (!st && t) *t = 0.0;
+    return st;
+}
+
+// ---- mesh
+
+int cupic_poisson_solver(cupic_run *r, double tol, int32_t *sweeps, double *resid)
+{
+    if (!r) return set_err(nullptr, CUPIC_ERR_ARG, "null run");
+    if (r->moved) return set_err(&r->err, CUPIC_ERR_STATE, "poisson_solver between mover and cc");
+    Target T = target_of(r);
+    seq_solve(T, 1, r->p.jacobi_mode, tol < 0 ? r->p.jacobi_tol : tol, r->p.jacobi_min_sweeps,
+              r->p.jacobi_max_sweeps, r->p.jacobi_check_every);
+    int st = check_launch(r, "poisson_solver");
+    if (st) return st;
+    if (sweeps || resid) {
+        RunState s;
+        st = read_state(r, &s);
+        if (sweeps) *sweeps = s.sweeps_done;
+        if (resid) *resid = s.last_resid;
+    }
+    return st;
+}
+
+int cupic_field_solver(cupic_run *r)
+{
+    if (!r) return set_err(nullptr, CUPIC_ERR_ARG, "null run");
+    if (r->moved) return set_err(&r->err, CUPIC_ERR_STATE, "field_solver between mover and cc");
+    seq_field(target_of(r));
+    return check_launch(r, "field_solver");
+}
+
+// ---- particles and boundary conditions
+
+int cupic_particle_mover(cupic_run *r)
+{
+    if (!r) return set_err(nullptr, CUPIC_ERR_ARG, "null run");
+    if (!r->initialised) return set_err(&r->err, CUPIC_ERR_STATE, "run not initialised");
+    if (r->moved) return set_err(&r->err, CUPIC_ERR_STATE, "particle_mover called twice");
+    if (r->acc_valid) {  // an unconsumed deposit would be added to
+        HIPTRY(&r->err, hipMemsetAsync(r->h.acc, 0, (size_t)r->h.nnodes * 8, r->stream));
+        r->acc_valid = false;
+    }
+    seq_mover(target_of(r));
+    r->moved = 1;
+    return check_launch(r, "particle_mover");
+}
+
+int cupic_cc(cupic_run *r, double t)
+{
+    if (!r) return set_err(nullptr, CUPIC_ERR_ARG, "null run");
+    if (!r->moved) return set_err(&r->err, CUPIC_ERR_STATE, "cc without particle_mover");
+    seq_cc(target_of(r), 1, t, 0);
+    r->moved = 0;
+    r->acc_valid = true;
+    r->t_next = t + r->p.dt;
+    return check_launch(r, "cc");
+}
+
+int cupic_step(cupic_run *r, double t)
+{
+    int st = cupic_charge_deposition(r);
+    if (!st) {
+        seq_solve(target_of(r), 0, r->p.jacobi_mode, 0, 0, r->p.jacobi_max_sweeps,
+                  r->p.jacobi_check_every);
+        seq_field(target_of(r));
+        st = check_launch(r, "solve/field");
+    }
+    if (!st) st = cupic_particle_mover(r);
+    if (!st) st = cupic_cc(r, t);
+    return st;
+}
+
+// ---- diagnostics
+
+int cupic_number_of_particles(cupic_run *r, int species, int64_t *n)
+{
+    if (!r || !n || species < 0 || species > 1) return set_err(nullptr, CUPIC_ERR_ARG, "bad argument");
+    RunState s;
+    int st = read_state(r, &s);
+    *n = s.N[species];
+    return st;
+}
+
+int cupic_probe_tally(cupic_run *r, int64_t last_probe[2], int64_t last_top[2],
+                      int64_t last_injected[2], int64_t total_probe[2], double *charge)
+{
+    if (!r) return set_err(nullptr, CUPIC_ERR_ARG, "null run");
+    RunState s;
+    int st = read_state(r, &s);
+    for (int k = 0; k < 2; k++) {
+        if (last_probe) last_probe[k] = (int64_t)s.last_probe[k];
+        if (last_top) last_top[k] = (int64_t)s.last_top[k];
+        if (last_injected) last_injected[k] = (int64_t)s.last_injected[k];
+        if (total_probe) total_probe[k] = (int64_t)s.total_probe[k];
+    }
+    if (charge) *charge = s.probe_charge_total;
+    return st;
+}
+
+int cupic_last_solve(cupic_run *r, int32_t *sweeps, double *resid)
+{
+    if (!r) return set_err(nullptr, CUPIC_ERR_ARG, "null run");
+    RunState s;
+    int st = read_state(r, &s);
+    if (sweeps) *sweeps = s.sweeps_done;
+    if (resid) *resid = s.last_resid;
+    return st;
+}
+
+int cupic_solver_totals(cupic_run *r, int64_t *sweeps_total, int64_t *solves)
+{
+    if (!r) return set_err(nullptr, CUPIC_ERR_ARG, "null run");
+    RunState s;
+    int st = read_state(r, &s);
+    if (sweeps_total) *sweeps_total = s.sweeps_total;
+    if (solves) *solves = s.solves;
+    return st;
+}
+
+int cupic_get_particles(cupic_run *r, int species, float *x, float *y, float *vx, float *vy,
+                        int64_t capacity, int64_t *n)
+{
+    if (!r || species < 0 || species > 1) return set_err(nullptr, CUPIC_ERR_ARG, "bad argument");
+    if (r->moved) return set_err(&r->err, CUPIC_ERR_STATE, "particles are pending cc");
+    RunState s;
+    int st = read_state(r, &s);
+    if (st && st != CUPIC_ERR_CAPACITY) return st;
+    int64_t N = s.N[species];
+    if (n) *n = N;
+    if (capacity < N) return set_err(&r->err, CUPIC_ERR_ARG, "host buffer too small");
+    int c = s.cur;
+    size_t b = (size_t)N * sizeof(float);
+    if (x) HIPTRY(&r->err, hipMemcpy(x, r->h.px[species][c], b, hipMemcpyDeviceToHost));
+    if (y) HIPTRY(&r->err, hipMemcpy(y, r->h.py[species][c], b, hipMemcpyDeviceToHost));
+    if (vx) HIPTRY(&r->err, hipMemcpy(vx, r->h.pvx[species][c], b, hipMemcpyDeviceToHost));
+    if (vy) HIPTRY(&r->err, hipMemcpy(vy, r->h.pvy[species][c], b, hipMemcpyDeviceToHost));
+    return st;
+}
+
+int cupic_set_particles(cupic_run *r, int species, const float *x, const float *y,
+                        const float *vx, const float *vy, int64_t n)
+{
+    if (!r || species < 0 || species > 1 || n < 0 || !x || !y || !vx || !vy)
+        return set_err(nullptr, CUPIC_ERR_ARG, "bad argument");
+    if (r->moved) return set_err(&r->err, CUPIC_ERR_STATE, "particles are pending cc");
+    if (n > r->h.cap) return set_err(&r->err, CUPIC_ERR_CAPACITY, "more particles than capacity");
+    const RunDev &h = r->h;
+    RunState s;
+    int st = read_state(r, &s);
+    if (st && st != CUPIC_ERR_CAPACITY) return st;
+    // stable counting sort by cell on the host
+    std::vector<int> key((size_t)n), cnt((size_t)h.ncells + 1, 0);
+    for (int64_t k = 0; k < n; k++) {
+        key[k] = host_cell(h, x[k], y[k]);
+        cnt[key[k] + 1]++;
+    }
+    for (int k = 0; k < h.ncells; k++) cnt[k + 1] += cnt[k];
+    std::vector<int> off(cnt);
+    std::vector<float> sx((size_t)n), sy((size_t)n), svx((size_t)n), svy((size_t)n);
+    for (int64_t k = 0; k < n; k++) {
+        int d = cnt[key[k]]++;
+        sx[d] = x[k]; sy[d] = y[k]; svx[d] = vx[k]; svy[d] = vy[k];
+    }
+    int c = s.cur;
+    size_t b = (size_t)n * sizeof(float);
+    HIPTRY(&r->err, hipMemcpy(h.px[species][c], sx.data(), b, hipMemcpyHostToDevice));
+    HIPTRY(&r->err, hipMemcpy(h.py[species][c], sy.data(), b, hipMemcpyHostToDevice));
+    HIPTRY(&r->err, hipMemcpy(h.pvx[species][c], svx.data(), b, hipMemcpyHostToDevice));
+    HIPTRY(&r->err, hipMemcpy(h.pvy[species][c], svy.data(), b, hipMemcpyHostToDevice));
+    HIPTRY(&r->err, hipMemcpy(h.off[species][c], off.data(), off.size() * sizeof(int),
+                              hipMemcpyHostToDevice));
+    s.N[species] = (int)n;
+    s.N_new[species] = (int)n;
+    r->acc_valid = false;
+    r->initialised = true;
+    return write_state(r, s);
+}
+
+int cupic_get_offsets(cupic_run *r, int species, int32_t *offsets)
+{
+    if (!r || !offsets || species < 0 || species > 1) return set_err(nullptr, CUPIC_ERR_ARG, "bad argument");
+    RunState s;
+    int st = read_state(r, &s);
+    if (st && st != CUPIC_ERR_CAPACITY) return st;
+    HIPTRY(&r->err, hipMemcpy(offsets, r->h.off[species][s.cur],
+                              ((size_t)r->h.ncells + 1) * sizeof(int), hipMemcpyDeviceToHost));
+    return st;
+}
+
+int cupic_get_mesh(cupic_run *r, int which, void *host, int64_t count)
+{
+    if (!r || !host) return set_err(nullptr, CUPIC_ERR_ARG, "bad argument");
+    const RunDev &h = r->h;
+    int64_t need = (int64_t)(h.ncx + 1) * (h.ncy + 1);
+    if (count < need) return set_err(&r->err, CUPIC_ERR_ARG, "host buffer too small");
+    RunState s;
+    int st = read_state(r, &s);
+    if (st && st != CUPIC_ERR_CAPACITY) return st;
+    size_t nn = (size_t)h.nnodes;
+    switch (which) {
+    case CUPIC_MESH_RHO: {
+        std::vector<float> v(nn);
+        HIPTRY(&r->err, hipMemcpy(v.data(), h.rho, nn * 4, hipMemcpyDeviceToHost));
+        to_reference(v, h.ncx, h.ncy, (float *)host);
+        break;
+    }
+    case CUPIC_MESH_PHI: {
+        std::vector<double> v(nn);
+        if (r->p.jacobi_mode == 2) {
+            HIPTRY(&r->err, hipMemcpy(v.data(), h.phi64, nn * 8, hipMemcpyDeviceToHost));
+        } else {
+            std::vector<float> f(nn);
+            HIPTRY(&r->err, hipMemcpy(f.data(), h.phi32[s.phi_cur], nn * 4, hipMemcpyDeviceToHost));
+            for (size_t k = 0; k < nn; k++) v[k] = f[k];
+        }
+        to_reference(v, h.ncx, h.ncy, (double *)host);
+        break;
+    }
+    case CUPIC_MESH_EX:
+    case CUPIC_MESH_EY: {
+        std::vector<float2> e(nn);
+        HIPTRY(&r->err, hipMemcpy(e.data(), h.E, nn * 8, hipMemcpyDeviceToHost));
+        std::vector<float> v(nn);
+        for (size_t k = 0; k < nn; k++) v[k] = which == CUPIC_MESH_EX ? e[k].x : e[k].y;
+        to_reference(v, h.ncx, h.ncy, (float *)host);
+        break;
+    }
+    case CUPIC_MESH_CHARGE: {
+        if (!r->acc_valid)
+            return set_err(&r->err, CUPIC_ERR_STATE,
+                           "no pending charge: it is consumed by charge_deposition");
+        std::vector<long long> v(nn);
+        HIPTRY(&r->err, hipMemcpy(v.data(), h.acc, nn * 8, hipMemcpyDeviceToHost));
+        to_reference(v, h.ncx, h.ncy, (long long *)host);
+        break;
+    }
+    default:
+        return set_err(&r->err, CUPIC_ERR_ARG, "unknown mesh quantity");
+    }
+    return CUPIC_OK;
+}
+
+int cupic_set_phi(cupic_run *r, const double *phi, int64_t count)
+{
+    if (!r || !phi) return set_err(nullptr, CUPIC_ERR_ARG, "bad argument");
+    const RunDev &h = r->h;
+    int nnx = h.ncx + 1;
+    if (count < (int64_t)nnx * (h.ncy + 1)) return set_err(&r->err, CUPIC_ERR_ARG, "array too small");
+    size_t nn = (size_t)h.nnodes;
+    std::vector<double> v(nn);
+    std::vector<float> f(nn);
+    for (int j = 0; j <= h.ncy; j++)
+        for (int i = 0; i < h.ncx; i++) {
+            v[(size_t)j * h.ncx + i] = phi[(size_t)j * nnx + i];
+            f[(size_t)j * h.ncx + i] = (float)phi[(size_t)j * nnx + i];
+        }
+    HIPTRY(&r->err, hipStreamSynchronize(r->stream));
+    HIPTRY(&r->err, hipMemcpy(h.phi64, v.data(), nn * 8, hipMemcpyHostToDevice));
+    HIPTRY(&r->err, hipMemcpy(h.phi32[0], f.data(), nn * 4, hipMemcpyHostToDevice));
+    HIPTRY(&r->err, hipMemcpy(h.phi32[1], f.data(), nn * 4, hipMemcpyHostToDevice));
+    return CUPIC_OK;
+}
+
+int cupic_get_clock(cupic_run *r, double tin[2], uint64_t injected_total[2], int32_t *started)
+{
+    if (!r) return set_err(nullptr, CUPIC_ERR_ARG, "null run");
+    RunState s;
+    int st = read_state(r, &s);
+    for (int k = 0; k < 2; k++) {
+        if (tin) tin[k] = s.tin[k];
+        if (injected_total) injected_total[k] = s.ninj[k];
+    }
+    if (started) *started = s.tin_started;
+    return st;
+}
+
+int cupic_set_clock(cupic_run *r, const double tin[2], const uint64_t injected_total[2],
+                    int32_t started)
+{
+    if (!r) return set_err(nullptr, CUPIC_ERR_ARG, "null run");
+    RunState s;
+    int st = read_state(r, &s);
+    if (st && st != CUPIC_ERR_CAPACITY) return st;
+    for (int k = 0; k < 2; k++) {
+        if (tin) s.tin[k] = tin[k];
+        if (injected_total) s.ninj[k] = injected_total[k];
+    }
+    s.tin_started = started;
+    return write_state(r, s);
+}
+
+// Text snapshots in the reference formats: " %.17e %.17e %.17e %.17e \n" per
+// particle (src/diagnostic.cu:76-79) and " %d %d %.17e \n" per node with a
+// blank line after each column (src/diagnostic.cu:113-120); ".dat" is
+// appended to the path as the reference does.
+int cupic_particles_snapshot(cupic_run *r, int species, const char *path, double t)
+{
+    (void)t;
+    if (!r || !path) return set_err(nullptr, CUPIC_ERR_ARG, "bad argument");
+    int64_t n = 0;
+    int st = cupic_number_of_particles(r, species, &n);
+    if (st) return st;
+    std::vector<float> x(n), y(n), vx(n), vy(n);
+    st = cupic_get_particles(r, species, x.data(), y.data(), vx.data(), vy.data(), n, &n);
+    if (st) return st;
+    std::string fn = std::string(path) + ".dat";
+    FILE *f = std::fopen(fn.c_str(), "w");
+    if (!f) return set_err(&r->err, CUPIC_ERR_IO, "cannot write " + fn);
+    for (int64_t k = 0; k < n; k++)
+        std::fprintf(f, " %.17e %.17e %.17e %.17e \n", (double)x[k], (double)y[k], (double)vx[k],
+                     (double)vy[k]);
+    std::fclose(f);
+    return CUPIC_OK;
+}
+
+int cupic_mesh_snapshot(cupic_run *r, int which, const char *path)
+{
+    if (!r || !path) return set_err(nullptr, CUPIC_ERR_ARG, "bad argument");
+    int nnx = r->h.ncx + 1, nny = r->h.ncy + 1;
+    size_t nn = (size_t)nnx * nny;
+    std::vector<double> v(nn);
+    int st;
+    if (which == CUPIC_MESH_PHI) {
+        st = cupic_get_mesh(r, which, v.data(), (int64_t)nn);
+    } else if (which == CUPIC_MESH_RHO || which == CUPIC_MESH_EX || which == CUPIC_MESH_EY) {
+        std::vector<float> f(nn);
+        st = cupic_get_mesh(r, which, f.data(), (int64_t)nn);
+        for (size_t k = 0; k < nn; k++) v[k] = f[k];
+    } else {
+        return set_err(&r->err, CUPIC_ERR_ARG, "unknown mesh quantity");
+    }
+    if (st) return st;
+    std::string fn = std::string(path) + ".dat";
+    FILE *f = std::fopen(fn.c_str(), "w");
+    if (!f) return set_err(&r->err, CUPIC_ERR_IO, "cannot write " + fn);
+    for (int i = 0; i < nnx; i++) {
+        for (int j = 0; j < nny; j++) std::fprintf(f, " %d %d %.17e \n", i, j, v[(size_t)i + (size_t)j * nnx]);
+        std::fprintf(f, "\n");
+    }
+    std::fclose(f);
+    return CUPIC_OK;
+}
+
+// ---- timing
+
+int cupic_timing_enable(cupic_run *r, int32_t on)
+{
+    if (!r) return set_err(nullptr, CUPIC_ERR_ARG, "null run");
+    r->timing = on != 0;
+    return CUPIC_OK;
+}
+
+int cupic_timing_read(cupic_run *r, double *ms, int64_t *launches, int32_t n)
+{
+    if (!r) return set_err(nullptr, CUPIC_ERR_ARG, "null run");
+    HIPTRY(&r->err, hipStreamSynchronize(r->stream));
+    for (auto &e : r->evs) {
+        float v = 0.0f;
+        if (hipEventElapsedTime(&v, e.a, e.b) == hipSuccess) {
+            r->t_ms[e.cls] += v;
+            r->t_n[e.cls] += 1;
+        }
+        r->pool.push_back(e.a);
+        r->pool.push_back(e.b);
+    }
+    r->evs.clear();
+    for (int k = 0; k < n && k < kTClasses; k++) {
+        if (ms) ms[k] = r->t_ms[k];
+        if (launches) launches[k] = r->t_n[k];
+        r->t_ms[k] = 0.0;
+        r->t_n[k] = 0;
+    }
+    return CUPIC_OK;
+}
+
+const char *cupic_timing_names(void) { return kTimeNames; }
+
+// ---- batches
+
+int cupic_batch_create(cupic_run *const *runs, int32_t nruns, int32_t use_graph,
+                       cupic_batch **out)
+{
+    if (!runs || nruns < 1 || !out) return set_err(nullptr, CUPIC_ERR_ARG, "bad argument");
+    *out = nullptr;
+    cupic_batch *b = new cupic_batch();
+    b->use_graph = use_graph != 0;
+    b->mode = runs[0]->p.jacobi_mode;
+    b->check = runs[0]->p.jacobi_check_every;
+    b->max_sweeps = runs[0]->p.jacobi_max_sweeps;
+    std::vector<RunDev> hs;
+    for (int k = 0; k < nruns; k++) {
+        cupic_run *r = runs[k];
+        if (!r || !r->initialised || r->moved) {
+            delete b;
+            return set_err(nullptr, CUPIC_ERR_STATE, "batch runs must be initialised and idle");
+        }
+        if (r->p.jacobi_mode != b->mode || r->p.jacobi_check_every != b->check ||
+            r->p.jacobi_max_sweeps != b->max_sweeps) {
+            delete b;
+            return set_err(nullptr, CUPIC_ERR_ARG, "batch runs must share the Jacobi schedule");
+        }
+        b->runs.push_back(r);
+        hs.push_back(r->h);
+    }
+    // every run enters the batch with its charge accumulated and its clock at
+    // the coming step's t
+    for (cupic_run *r : b->runs) {
+        if (!r->acc_valid) {
+            HIPTRY(nullptr, hipMemsetAsync(r->h.acc, 0, (size_t)r->h.nnodes * 8, r->stream));
+            launch_deposit(target_of(r).L(kShapeTasks));
+            r->acc_valid = true;
+        }
+        RunState s;
+        int st = read_state(r, &s);
+        if (st) { delete b; return st; }
+        s.t = r->t_next;
+        st = write_state(r, s);
+        if (st) { delete b; return st; }
+    }
+    std::vector<int> plan((size_t)kShapeCount * (nruns + 1));
+    for (int s = 0; s < kShapeCount; s++) {
+        int acc = 0;
+        for (int k = 0; k < nruns; k++) {
+            plan[(size_t)s * (nruns + 1) + k] = acc;
+            acc += shape_blocks(hs[k], (Shape)s);
+        }
+        plan[(size_t)s * (nruns + 1) + nruns] = acc;
+        b->nblk[s] = acc;
+    }
+    if (hipMalloc(&b->d_rds, hs.size() * sizeof(RunDev)) != hipSuccess ||
+        hipMalloc(&b->d_plan, plan.size() * sizeof(int)) != hipSuccess) {
+        cupic_batch_destroy(b);
+        return set_err(nullptr, CUPIC_ERR_ALLOC, "batch allocation failed");
+    }
+    HIPTRY(nullptr, hipMemcpy(b->d_rds, hs.data(), hs.size() * sizeof(RunDev), hipMemcpyHostToDevice));
+    HIPTRY(nullptr, hipMemcpy(b->d_plan, plan.data(), plan.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIPTRY(nullptr, hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    HIPTRY(nullptr, hipDeviceSynchronize());
+    if (b->use_graph) {
+        Target T{b->d_rds, b->d_plan, b->nblk, nruns, b->stream, nullptr};
+        HIPTRY(&b->err, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        launch_rho_convert(T.L(kShapeNodes));
+        seq_solve(T, 0, b->mode, 0, 0, b->max_sweeps, b->check);
+        launch_field(T.L(kShapeNodes));
+        launch_push_rank(T.L(kShapeTasks));
+        seq_cc(T, 0, 0.0, 1);
+        HIPTRY(&b->err, hipStreamEndCapture(b->stream, &b->graph));
+        HIPTRY(&b->err, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    }
+    *out = b;
+    return CUPIC_OK;
+}
+
+int cupic_batch_step(cupic_batch *b, int32_t nsteps)
+{
+    if (!b) return set_err(nullptr, CUPIC_ERR_ARG, "null batch");
+    int nruns = (int)b->runs.size();
+    Target T{b->d_rds, b->d_plan, b->nblk, nruns, b->stream, nullptr};
+    for (int k = 0; k < nsteps; k++) {
+        if (b->use_graph) {
+            HIPTRY(&b->err, hipGraphLaunch(b->exec, b->stream));
+        } else {
+            launch_rho_convert(T.L(kShapeNodes));
+            seq_solve(T, 0, b->mode, 0, 0, b->max_sweeps, b->check);
+            launch_field(T.L(kShapeNodes));
+            launch_push_rank(T.L(kShapeTasks));
+            seq_cc(T, 0, 0.0, 1);
+        }
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+        return set_err(&b->err, CUPIC_ERR_HIP, std::string("batch step: ") + hipGetErrorString(e));
+    HIPTRY(&b->err, hipStreamSynchronize(b->stream));
+    for (cupic_run *r : b->runs) {
+        r->t_next += nsteps * r->p.dt;
+        r->acc_valid = true;
+    }
+    return CUPIC_OK;
+}
+
+int cupic_batch_destroy(cupic_batch *b)
+{
+    if (!b) return CUPIC_OK;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    if (b->d_rds) (void)hipFree(b->d_rds);
+    if (b->d_plan) (void)hipFree(b->d_plan);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return CUPIC_OK;
+}
+
+const char *cupic_batch_last_error(const cupic_batch *b)
+{
+    if (b && !b->err.empty()) return b->err.c_str();
+    return g_err.c_str();
+}
+
+}  // extern "C"

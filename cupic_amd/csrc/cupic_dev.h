@@ -1,0 +1,293 @@
+// cupic_dev.h -- device-side data layout and shared device functions.
+//
+// Layout in HBM (per run):
+//   particles   FP32 SoA x, y, vx, vy per species, two buffers (ping-pong),
+//               sorted by cell key jc*ncx + ic with per-cell offsets.
+//   grids       periodic in x, [ncy+1][ncx]: node (i, j) at j*ncx + i.
+//               charge: int64 fixed point (2^-32 e); rhs: FP32 and FP64;
+//               phi: FP32 ping-pong (modes 0, 1) / FP64 master + FP32
+//               correction ping-pong (mode 2); E: float2 (Ex, Ey).
+//   sort meta   cnt[old cell][9 directions], base[new cell][10 slots],
+//               far list of particles that moved more than one cell or were
+//               injected.
+//
+// Every kernel is written for a batch of runs: blockIdx.x is mapped to
+// (run, local block) through a prefix array, so the plain path is a batch
+// of one.  All run-dependent control state lives in device memory
+// (RunState) so a whole step can be captured in a HIP graph.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace cupic {
+
+constexpr int kWave = 64;
+constexpr int kBlock = 256;
+constexpr int kCmax = 16;              // max cells per wave task
+constexpr int kTileW = kCmax + 3;      // deposit tile width (node columns)
+constexpr int kScanCells = 1024;       // cells per scan block
+
+enum Dir : int { kDirFar = 9, kDirProbe = 10, kDirTop = 11, kDirNone = 15 };
+
+struct FarRec {             // 32 B
+    int32_t key;            // new cell
+    int32_t idx;            // input index (injected: N + k)
+    int32_t rank;           // stable rank within (key, far slot)
+    int32_t pad;
+    float x, y, vx, vy;     // final state
+};
+
+struct SpecConst {
+    float qdtm;             // q dt / m
+    float hkick;            // -q dt / (2 m)
+    double q, m, sigma, dtin;
+};
+
+// Mutable per-run control state (device memory).
+struct RunState {
+    int cur;                        // current particle buffer
+    int phi_cur;                    // current FP32 phi buffer (modes 0, 1)
+    int N[2];                       // particles per species (current buffer)
+    int N_new[2];
+    int n_in[2];                    // injected this step
+    int inj_base[2];                // first far-list slot of injected particles
+    int far_n[2];                   // far-list length this step
+    int tin_started;
+    int step;
+    double t;
+    double tin[2];
+    unsigned long long ninj[2];     // injected so far (RNG sequence)
+    unsigned long long ninj_first[2];
+    // solver
+    int solve_mode, solve_min, solve_max, solve_check;
+    double solve_tol;
+    int solve_done;
+    int sweeps_done;
+    double last_resid;
+    long long sweeps_total;         // sweeps over all solves
+    long long solves;
+    // tallies (this step and running)
+    unsigned long long probe[2], top[2];
+    unsigned long long last_probe[2], last_top[2], last_injected[2];
+    unsigned long long total_probe[2], total_top[2], total_injected[2];
+    double probe_charge_total;
+    unsigned int err;               // error bits, see kErr*
+};
+
+enum : unsigned int {
+    kErrFarOverflow = 1u,
+    kErrCapacity = 2u,
+    kErrInjectCap = 4u,
+    kErrSortInvariant = 8u,
+};
+
+// Immutable per-run description (device memory).
+struct RunDev {
+    int ncx, ncy, ncells, nnodes;
+    int C, tpr;                     // cells per wave task, tasks per row
+    int cap;                        // particles per species per buffer
+    int far_cap, max_inject;
+    int n_init;                     // initial particles per species
+    int per_row;
+    int max_sweeps_alloc;           // size of the per-sweep reduction slots
+    int step_mode, step_min, step_max, step_check;
+    double step_tol, init_tol;
+    float inv_ds, Lx_f, Ly_f, dt_f;
+    double ds, dt, Lx, Ly, eps0, phi_p;
+    double b_interior;              // 2^-32 / (ds eps0): charge units -> rhs
+    double rho_interior, rho_edge;  // 2^-32 / ds^3, 2^-32 / (ds^3 / 2)
+    SpecConst sp[2];
+    unsigned long long seed, run_id;
+    unsigned int quirks;
+    float *px[2][2], *py[2][2], *pvx[2][2], *pvy[2][2];   // [species][buffer]
+    int *off[2][2];                 // [species][buffer], ncells + 1
+    int *cnt[2];                    // ncells * 9
+    int *base[2];                   // ncells * 10
+    int *far_cnt[2];                // ncells
+    FarRec *far[2];                 // far_cap
+    double *inj_tin[2];             // max_inject
+    int *tot[2];                    // ncells
+    int *blk[2];                    // scan block sums / offsets
+    unsigned long long *acc;        // nnodes, int64 charge
+    float *rho;                     // nnodes
+    float *b32;                     // nnodes, Jacobi rhs
+    double *b64;                    // nnodes
+    float *phi32[2];                // nnodes, modes 0/1
+    double *phi64;                  // nnodes, mode 2 master
+    float *d32[2];                  // nnodes, mode 2 correction
+    float *r32;                     // nnodes, mode 2 correction rhs
+    float2 *E;                      // nnodes
+    unsigned int *sweep_max;        // max_sweeps_alloc + 1 (float bits)
+    unsigned long long *resid;      // max_sweeps_alloc + 2 (double bits)
+    RunState *st;
+};
+
+// ---------------------------------------------------------------- helpers
+
+__device__ __forceinline__ int find_run(const int *__restrict__ start, int nruns, int b)
+{
+    int lo = 0, hi = nruns - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (start[mid] <= b) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ float wave_max_f(float v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+__device__ __forceinline__ double wave_max_d(double v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// ---------------------------------------------------------------- Philox-4x64-10
+
+__device__ __forceinline__ void philox4x64_10(unsigned long long c[4], unsigned long long k0,
+                                              unsigned long long k1)
+{
+#pragma unroll
+    for (int r = 0; r < 10; r++) {
+        unsigned long long hi0 = __umul64hi(0xD2E7470EE14C6C93ULL, c[0]);
+        unsigned long long lo0 = 0xD2E7470EE14C6C93ULL * c[0];
+        unsigned long long hi1 = __umul64hi(0xCA5A826395121157ULL, c[2]);
+        unsigned long long lo1 = 0xCA5A826395121157ULL * c[2];
+        unsigned long long n0 = hi1 ^ c[1] ^ k0;
+        unsigned long long n2 = hi0 ^ c[3] ^ k1;
+        c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;
+        k0 += 0x9E3779B97F4A7C15ULL;
+        k1 += 0xBB67AE8584CAA73BULL;
+    }
+}
+
+// counter (index, species | purpose << 8, 0, 0), key (seed, run id)
+__device__ __forceinline__ void draw4(const RunDev &R, int species, int purpose,
+                                      unsigned long long index, unsigned long long out[4])
+{
+    out[0] = index;
+    out[1] = (unsigned long long)species | ((unsigned long long)purpose << 8);
+    out[2] = 0;
+    out[3] = 0;
+    philox4x64_10(out, R.seed, R.run_id);
+}
+
+__device__ __forceinline__ double u01_open(unsigned long long r)
+{
+    return ((double)(r >> 12) + 0.5) * 0x1p-52;
+}
+
+// ---------------------------------------------------------------- cell keys
+
+// Cell of an FP32 position and its in-cell fractions: one product, one
+// floor and clamps, evaluated without contraction.
+__device__ __forceinline__ void cell_of(const RunDev &R, float x, float y, float &fx, float &fy,
+                                        int &ic, int &jc)
+{
+    float tx = x * R.inv_ds;
+    float ty = y * R.inv_ds;
+    float bx = fminf(fmaxf(floorf(tx), 0.0f), (float)(R.ncx - 1));
+    float by = fminf(fmaxf(floorf(ty), 0.0f), (float)(R.ncy - 1));
+    fx = fminf(fmaxf(tx - bx, 0.0f), 1.0f);
+    fy = fminf(fmaxf(ty - by, 0.0f), 1.0f);
+    ic = (int)bx;
+    jc = (int)by;
+}
+
+// CIC gather from interleaved (Ex, Ey), fixed operation order.
+__device__ __forceinline__ void gather(const RunDev &R, const float2 *__restrict__ E, float fx,
+                                       float fy, int ic, int jc, float &ex, float &ey)
+{
+    int i1 = (ic + 1 == R.ncx) ? 0 : ic + 1;
+    int k00 = jc * R.ncx + ic, k10 = jc * R.ncx + i1;
+    int k01 = k00 + R.ncx, k11 = k10 + R.ncx;
+    float2 e00 = E[k00], e10 = E[k10], e01 = E[k01], e11 = E[k11];
+    float gx = 1.0f - fx, gy = 1.0f - fy;
+    float w00 = gx * gy, w10 = fx * gy, w01 = gx * fy, w11 = fx * fy;
+    float a = w00 * e00.x;
+    a = a + w10 * e10.x;
+    a = a + w01 * e01.x;
+    a = a + w11 * e11.x;
+    float b = w00 * e00.y;
+    b = b + w10 * e10.y;
+    b = b + w01 * e01.y;
+    b = b + w11 * e11.y;
+    ex = a;
+    ey = b;
+}
+
+struct Pushed {
+    float x, y, vx, vy;
+    float fx, fy;      // fractions in the new cell
+    int oic, ojc;      // old cell
+    int nic, njc;      // new cell
+    int di;            // x displacement in cells (-1, 0, 1 for near movers)
+    int dir;           // 0..8 near, kDirFar, kDirProbe, kDirTop
+};
+
+// Leapfrog push of one particle (src/particles.cu:110-231 semantics),
+// periodic wrap into [0, Lx) (src/cc.cu:775-810, Q6), absorption at y < 0
+// (probe) and y >= Ly (top) (src/cc.cu:182-185), new cell and direction.
+__device__ __forceinline__ Pushed push_one(const RunDev &R, const float2 *__restrict__ E,
+                                           float qdtm, float x, float y, float vx, float vy)
+{
+    Pushed o;
+    float fx, fy;
+    int oic, ojc;
+    cell_of(R, x, y, fx, fy, oic, ojc);
+    float ex, ey;
+    gather(R, E, fx, fy, oic, ojc, ex, ey);
+    float nvx = vx + qdtm * ex;
+    float nvy = vy + qdtm * ey;
+    float nx = x + R.dt_f * nvx;
+    float ny = y + R.dt_f * nvy;
+    if (nx < 0.0f) nx = nx + R.Lx_f;
+    if (nx >= R.Lx_f) nx = nx - R.Lx_f;
+    o.x = nx; o.y = ny; o.vx = nvx; o.vy = nvy;
+    o.oic = oic; o.ojc = ojc;
+    o.fx = 0.0f; o.fy = 0.0f; o.nic = 0; o.njc = 0; o.di = 0;
+    float ty = ny * R.inv_ds;
+    if (ty < 0.0f) { o.dir = kDirProbe; return o; }
+    if (ty >= (float)R.ncy) { o.dir = kDirTop; return o; }
+    int nic, njc;
+    cell_of(R, nx, ny, fx, fy, nic, njc);
+    o.fx = fx; o.fy = fy; o.nic = nic; o.njc = njc;
+    int di = nic - oic, dj = njc - ojc;
+    if (di == R.ncx - 1) di = -1;
+    else if (di == -(R.ncx - 1)) di = 1;
+    o.di = di;
+    if (di >= -1 && di <= 1 && dj >= -1 && dj <= 1)
+        o.dir = (di + 1) + 3 * (dj + 1);
+    else
+        o.dir = kDirFar;
+    return o;
+}
+
+// 16-bit-per-axis fixed-point weights; every particle carries 2^32 units.
+__device__ __forceinline__ void fixed_weights(float fx, float fy, unsigned &wx, unsigned &wy)
+{
+    wx = (unsigned)rintf(fx * 65536.0f);
+    wy = (unsigned)rintf(fy * 65536.0f);
+}
+
+}  // namespace cupic

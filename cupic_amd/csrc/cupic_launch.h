@@ -1,0 +1,68 @@
+// cupic_launch.h -- host-side launchers for the device kernels.
+//
+// A Launch names a batch of runs (device array of RunDev), the prefix array
+// that maps blockIdx.x to (run, local block) for the kernel's work shape, and
+// the stream.  The plain path passes a batch of one.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "cupic_dev.h"
+
+namespace cupic {
+
+struct Launch {
+    const RunDev *rds;     // device array, nruns entries
+    const int *start;      // device array, nruns + 1 block prefixes
+    int nruns;
+    int nblocks;
+    hipStream_t stream;
+};
+
+// Work shapes: blocks per run for each kernel family.
+enum Shape : int {
+    kShapeNodes = 0,   // ceil(nnodes / 256)
+    kShapeTasks,       // ceil(2 * ncy * tpr / 4): one wave per (species, row, column block)
+    kShapeCellBlk,     // 2 * ceil(ncells / kScanCells)
+    kShapeRunSp,       // 2: one block per species
+    kShapeRun,         // 1
+    kShapeFar,         // 2 * ceil(far_cap / 256)
+    kShapeInject,      // 2 * ceil(max_inject / 256)
+    kShapeRows,        // 2 * ncy
+    kShapeInit,        // 2 * ceil(n_init / 256)
+    kShapeCount
+};
+
+int shape_blocks(const RunDev &h, Shape s);
+
+// mesh
+void launch_phi_linear(const Launch &L);
+void launch_rho_convert(const Launch &L);
+void launch_solver_begin(const Launch &L, int use_args, int mode, double tol, int min_sweeps,
+                         int max_sweeps, int check_every);
+void launch_jacobi_sweep(const Launch &L, int s, int mode, int check_slot, int first);
+void launch_ir_update(const Launch &L, int s);
+void launch_residual(const Launch &L, int c, int s, int mode);
+void launch_solver_end(const Launch &L, int last_check);
+void launch_field(const Launch &L);
+
+// particles
+void launch_create(const Launch &L);
+void launch_init_sort(const Launch &L);
+void launch_deposit(const Launch &L);
+void launch_half_kick(const Launch &L);
+void launch_push_rank(const Launch &L);
+void launch_scatter(const Launch &L);
+
+// boundary conditions and sort
+void launch_clock(const Launch &L, int use_host_t, double t_host);
+void launch_inject(const Launch &L);
+void launch_far_rank(const Launch &L);
+void launch_scan_a(const Launch &L);
+void launch_scan_b(const Launch &L);
+void launch_scan_c(const Launch &L);
+void launch_far_scatter(const Launch &L);
+void launch_finish(const Launch &L, int advance_t);
+void launch_init_finish(const Launch &L);
+
+}  // namespace cupic

@@ -1,0 +1,403 @@
+// k_cc.hip -- boundary conditions and the cell-key sort bookkeeping:
+// injection clock and flux-Maxwellian injection at y = Ly, ranks of the far
+// list (particles that moved more than one cell, and injected ones), the
+// per-cell scan that turns direction counts into new offsets and slot
+// bases, the far-list scatter and the end-of-step bookkeeping (buffer flip,
+// probe tally).
+#include "cupic_launch.h"
+
+namespace cupic {
+
+#define RUN_PROLOGUE()                                              \
+    const int b_ = blockIdx.x;                                      \
+    const int r_ = find_run(start, nruns, b_);                      \
+    const RunDev &R = rds[r_];                                      \
+    const int local = b_ - start[r_];                               \
+    RunState *st = R.st;                                            \
+    (void)local; (void)st;
+
+constexpr double kPi = 3.1415926535897932;  // src/stdh.h:26
+
+// Injection clock (src/cc.cu:35-36, 147, 179, 247): tin starts at the first
+// cc call as t + dtin; this step inserts in = 1 + int((t+dt-tin)/dtin)
+// particles when tin < t+dt, advancing tin by dtin once per particle.  The
+// recurrence runs serially in FP64 so the counts match the reference's
+// host loop bit for bit.
+__global__ __launch_bounds__(kBlock) void k_clock(const RunDev *rds, const int *start,
+                                                  int nruns, int use_host_t, double t_host)
+{
+    RUN_PROLOGUE();
+    if (threadIdx.x == 0) {
+        if (use_host_t) st->t = t_host;
+        if (!st->tin_started) {
+            st->tin[0] = st->t + R.sp[0].dtin;
+            st->tin[1] = st->t + R.sp[1].dtin;
+            st->tin_started = 1;
+        }
+    }
+    __syncthreads();
+    int s = threadIdx.x;
+    if (s > 1) return;
+    double t = st->t;
+    double fpt = t + R.dt;
+    double tin = st->tin[s];
+    double dtin = R.sp[s].dtin;
+    int n = 0;
+    if (tin < fpt) n = 1 + (int)((fpt - tin) / dtin);
+    double *seq = R.inj_tin[s];
+    for (int k = 0; k < n; k++) {
+        if (k < R.max_inject) seq[k] = tin;
+        tin += dtin;
+    }
+    st->tin[s] = tin;
+    int ne = n;
+    unsigned int err = 0;
+    if (ne > R.max_inject) { ne = R.max_inject; err |= kErrInjectCap; }
+    if (st->N[s] + ne > R.cap) { ne = max(0, R.cap - st->N[s]); err |= kErrCapacity; }
+    int fb = st->far_n[s];
+    if (fb + ne > R.far_cap) { ne = max(0, R.far_cap - fb); err |= kErrFarOverflow; }
+    if (err) atomicOr(&st->err, err);
+    st->n_in[s] = ne;
+    st->inj_base[s] = fb;
+    st->far_n[s] = fb + ne;
+    st->ninj_first[s] = st->ninj[s];
+    st->ninj[s] += (unsigned long long)ne;
+}
+
+// One injected particle (src/cc.cu:214-247), evaluated in FP64: x = U Lx,
+// y = Ly, vx ~ N(0, sigma), vy = -Rayleigh(sigma); E interpolated from the
+// top node row (disty = 1); partial push x += (t+dt-tin) v and
+// v += q (t+dt/2-tin) E/m (Q7; the committed sign for both species is kept
+// behind a quirk flag); wrapped in x, rounded to FP32, wrapped again.
+__global__ __launch_bounds__(kBlock) void k_inject(const RunDev *rds, const int *start,
+                                                   int nruns)
+{
+    RUN_PROLOGUE();
+    int bps = (R.max_inject + kBlock - 1) / kBlock;
+    int s = local / bps;
+    int k = (local - s * bps) * kBlock + threadIdx.x;
+    if (s > 1 || k >= st->n_in[s]) return;
+    double tin = R.inj_tin[s][k];
+    unsigned long long r[4];
+    draw4(R, s, 1, st->ninj_first[s] + (unsigned long long)k, r);
+    const SpecConst &S = R.sp[s];
+    double t = st->t;
+    double fpt = t + R.dt, fvt = t + 0.5 * R.dt;
+    double x = u01_open(r[0]) * R.Lx;
+    double y = R.Ly;
+    double g = sqrt(-2.0 * log(u01_open(r[1]))) * cos(2.0 * kPi * u01_open(r[2]));
+    double vx = S.sigma * g;
+    double vy = -(S.sigma * sqrt(-2.0 * log(u01_open(r[3]))));
+    int ic = (int)(x / R.ds);
+    if (ic > R.ncx - 1) ic = R.ncx - 1;
+    int jc = R.ncy - 1;
+    int i1 = ic + 1 == R.ncx ? 0 : ic + 1;
+    double distx = fabs((double)ic * R.ds - x) / R.ds;
+    double disty = 1.0;
+    float2 e00 = R.E[jc * R.ncx + ic], e10 = R.E[jc * R.ncx + i1];
+    float2 e01 = R.E[(jc + 1) * R.ncx + ic], e11 = R.E[(jc + 1) * R.ncx + i1];
+    double epx = (double)e00.x * (1.0 - distx) * (1.0 - disty);
+    epx += (double)e10.x * distx * (1.0 - disty);
+    epx += (double)e01.x * (1.0 - distx) * disty;
+    epx += (double)e11.x * distx * disty;
+    double epy = (double)e00.y * (1.0 - distx) * (1.0 - disty);
+    epy += (double)e10.y * distx * (1.0 - disty);
+    epy += (double)e01.y * (1.0 - distx) * disty;
+    epy += (double)e11.y * distx * disty;
+    x += (fpt - tin) * vx;
+    y += (fpt - tin) * vy;
+    if ((R.quirks & 1u) || S.q < 0.0) {
+        vx -= (fvt - tin) * epx / S.m;
+        vy -= (fvt - tin) * epy / S.m;
+    } else {
+        vx += (fvt - tin) * epx / S.m;
+        vy += (fvt - tin) * epy / S.m;
+    }
+    if (x < 0.0) x += R.Lx;
+    if (x >= R.Lx) x -= R.Lx;
+    float xf = (float)x;
+    if (xf < 0.0f) xf = xf + R.Lx_f;
+    if (xf >= R.Lx_f) xf = xf - R.Lx_f;
+    FarRec rec;
+    rec.x = xf;
+    rec.y = (float)y;
+    rec.vx = (float)vx;
+    rec.vy = (float)vy;
+    float fx, fy;
+    int nic, njc;
+    cell_of(R, rec.x, rec.y, fx, fy, nic, njc);
+    rec.key = njc * R.ncx + nic;
+    rec.idx = st->N[s] + k;
+    rec.rank = 0;
+    rec.pad = 0;
+    R.far[s][st->inj_base[s] + k] = rec;
+}
+
+// Stable rank of each far-list entry within its new cell: the number of
+// entries with the same cell and a smaller input index.  The list is short
+// (injected particles plus rare fast movers), so entries are compared
+// directly, 256 at a time through LDS; far_cnt counts entries per cell.
+__global__ __launch_bounds__(kBlock) void k_far_rank(const RunDev *rds, const int *start,
+                                                     int nruns)
+{
+    RUN_PROLOGUE();
+    __shared__ int skey[kBlock];
+    __shared__ int sidx[kBlock];
+    int bps = (R.far_cap + kBlock - 1) / kBlock;
+    int s = local / bps;
+    if (s > 1) return;
+    int e0 = (local - s * bps) * kBlock;
+    int F = min(st->far_n[s], R.far_cap);
+    if (e0 >= F) return;  // block-uniform
+    int e = e0 + threadIdx.x;
+    FarRec *far = R.far[s];
+    int key = -1, idx = 0;
+    if (e < F) { key = far[e].key; idx = far[e].idx; }
+    int rank = 0;
+    for (int t0 = 0; t0 < F; t0 += kBlock) {
+        int q = t0 + threadIdx.x;
+        __syncthreads();
+        skey[threadIdx.x] = q < F ? far[q].key : -2;
+        sidx[threadIdx.x] = q < F ? far[q].idx : 0;
+        __syncthreads();
+        int m = min(kBlock, F - t0);
+        for (int u = 0; u < m; u++) rank += (skey[u] == key && sidx[u] < idx);
+    }
+    if (e < F) {
+        far[e].rank = rank;
+        atomicAdd(&R.far_cnt[s][key], 1);
+    }
+}
+
+// Count of particles arriving in new cell k from neighbour slot sl (source
+// offset (sl%3-1, sl/3-1), periodic in x): the source cell's count for the
+// direction that leads to k.
+__device__ __forceinline__ int src_count(const RunDev &R, const int *cnt, int i, int j, int sl)
+{
+    int dj = sl / 3 - 1, di = sl - 3 * (sl / 3) - 1;
+    int jj = j + dj;
+    if (jj < 0 || jj >= R.ncy) return 0;
+    int ii = i + di;
+    if (ii < 0) ii += R.ncx;
+    if (ii >= R.ncx) ii -= R.ncx;
+    return cnt[(jj * R.ncx + ii) * 9 + (8 - sl)];
+}
+
+__device__ __forceinline__ int block_excl_scan(int v, int *sh, int &total)
+{
+    // Hillis-Steele over 256 entries in LDS
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 1; o < kBlock; o <<= 1) {
+        int a = threadIdx.x >= (unsigned)o ? sh[threadIdx.x - o] : 0;
+        __syncthreads();
+        sh[threadIdx.x] += a;
+        __syncthreads();
+    }
+    total = sh[kBlock - 1];
+    int incl = sh[threadIdx.x];
+    __syncthreads();
+    return incl - v;
+}
+
+// Per-cell totals of the new state and per-block sums.
+__global__ __launch_bounds__(kBlock) void k_scan_a(const RunDev *rds, const int *start,
+                                                   int nruns)
+{
+    RUN_PROLOGUE();
+    __shared__ int sh[kBlock];
+    int nblk = (R.ncells + kScanCells - 1) / kScanCells;
+    int s = local / nblk;
+    int blk = local - s * nblk;
+    int sum = 0;
+    for (int u = 0; u < kScanCells / kBlock; u++) {
+        int k = blk * kScanCells + threadIdx.x * (kScanCells / kBlock) + u;
+        if (k >= R.ncells) break;
+        int j = k / R.ncx, i = k - j * R.ncx;
+        int t = R.far_cnt[s][k];
+#pragma unroll
+        for (int sl = 0; sl < 9; sl++) t += src_count(R, R.cnt[s], i, j, sl);
+        R.tot[s][k] = t;
+        sum += t;
+    }
+    int total;
+    block_excl_scan(sum, sh, total);
+    if (threadIdx.x == 0) R.blk[s][blk] = total;
+}
+
+// Exclusive scan of the block sums of one species; N_new = total.
+__global__ __launch_bounds__(kBlock) void k_scan_b(const RunDev *rds, const int *start,
+                                                   int nruns)
+{
+    RUN_PROLOGUE();
+    __shared__ int sh[kBlock];
+    int s = local;
+    if (s > 1) return;
+    int nblk = (R.ncells + kScanCells - 1) / kScanCells;
+    int carry = 0;
+    for (int b0 = 0; b0 < nblk; b0 += kBlock) {
+        int q = b0 + threadIdx.x;
+        int v = q < nblk ? R.blk[s][q] : 0;
+        int total;
+        int ex = block_excl_scan(v, sh, total);
+        if (q < nblk) R.blk[s][q] = carry + ex;
+        carry += total;
+    }
+    if (threadIdx.x == 0) st->N_new[s] = carry;
+}
+
+// New offsets and slot bases: base[k*10+sl] = off[k] + arrivals from slots
+// before sl; slot 9 (far list) last.  Clears far_cnt for the next step.
+__global__ __launch_bounds__(kBlock) void k_scan_c(const RunDev *rds, const int *start,
+                                                   int nruns)
+{
+    RUN_PROLOGUE();
+    __shared__ int sh[kBlock];
+    int nblk = (R.ncells + kScanCells - 1) / kScanCells;
+    int s = local / nblk;
+    int blk = local - s * nblk;
+    constexpr int per = kScanCells / kBlock;
+    int k0 = blk * kScanCells + threadIdx.x * per;
+    int sum = 0;
+    for (int u = 0; u < per; u++) {
+        int k = k0 + u;
+        if (k < R.ncells) sum += R.tot[s][k];
+    }
+    int total;
+    int run = block_excl_scan(sum, sh, total) + R.blk[s][blk];
+    int *noff = R.off[s][st->cur ^ 1];
+    int *base = R.base[s];
+    for (int u = 0; u < per; u++) {
+        int k = k0 + u;
+        if (k >= R.ncells) break;
+        int j = k / R.ncx, i = k - j * R.ncx;
+        noff[k] = run;
+        int b = run;
+#pragma unroll
+        for (int sl = 0; sl < 9; sl++) {
+            base[k * 10 + sl] = b;
+            b += src_count(R, R.cnt[s], i, j, sl);
+        }
+        base[k * 10 + 9] = b;
+        R.far_cnt[s][k] = 0;
+        run += R.tot[s][k];
+    }
+    if (blk == nblk - 1 && threadIdx.x == kBlock - 1) noff[R.ncells] = st->N_new[s];
+}
+
+// Far-list entries to their sorted positions, with their charge (global
+// int64 atomics; the list is short).
+__global__ __launch_bounds__(kBlock) void k_far_scatter(const RunDev *rds, const int *start,
+                                                        int nruns)
+{
+    RUN_PROLOGUE();
+    int bps = (R.far_cap + kBlock - 1) / kBlock;
+    int s = local / bps;
+    if (s > 1) return;
+    int e = (local - s * bps) * kBlock + threadIdx.x;
+    int F = min(st->far_n[s], R.far_cap);
+    if (e >= F) return;
+    FarRec rec = R.far[s][e];
+    int o = st->cur ^ 1;
+    int dest = R.base[s][rec.key * 10 + 9] + rec.rank;
+    if ((unsigned)dest >= (unsigned)R.cap) {
+        atomicOr(&st->err, kErrSortInvariant);
+        return;
+    }
+    R.px[s][o][dest] = rec.x;
+    R.py[s][o][dest] = rec.y;
+    R.pvx[s][o][dest] = rec.vx;
+    R.pvy[s][o][dest] = rec.vy;
+    float fx, fy;
+    int ic, jc;
+    cell_of(R, rec.x, rec.y, fx, fy, ic, jc);
+    unsigned wx, wy;
+    fixed_weights(fx, fy, wx, wy);
+    long long sgn = s == 0 ? -1 : 1;
+    long long gx = 65536 - (long long)wx, gy = 65536 - (long long)wy;
+    int i1 = ic + 1 == R.ncx ? 0 : ic + 1;
+    atomicAdd(&R.acc[jc * R.ncx + ic], (unsigned long long)(sgn * gx * gy));
+    atomicAdd(&R.acc[jc * R.ncx + i1], (unsigned long long)(sgn * (long long)wx * gy));
+    atomicAdd(&R.acc[(jc + 1) * R.ncx + ic], (unsigned long long)(sgn * gx * (long long)wy));
+    atomicAdd(&R.acc[(jc + 1) * R.ncx + i1],
+              (unsigned long long)(sgn * (long long)wx * (long long)wy));
+}
+
+// End of cc: flip buffers, commit counts and the probe tally (probe charge
+// accumulated in FP64: ions +1, electrons -1).
+__global__ __launch_bounds__(kBlock) void k_finish(const RunDev *rds, const int *start,
+                                                   int nruns, int advance_t)
+{
+    RUN_PROLOGUE();
+    if (threadIdx.x != 0) return;
+    double dq = 0.0;
+    for (int s = 0; s < 2; s++) {
+        st->last_probe[s] = st->probe[s];
+        st->last_top[s] = st->top[s];
+        st->last_injected[s] = (unsigned long long)st->n_in[s];
+        st->total_probe[s] += st->probe[s];
+        st->total_top[s] += st->top[s];
+        st->total_injected[s] += (unsigned long long)st->n_in[s];
+        dq += R.sp[s].q * (double)st->probe[s];
+        st->probe[s] = 0;
+        st->top[s] = 0;
+        st->far_n[s] = 0;
+        st->n_in[s] = 0;
+        st->N[s] = st->N_new[s];
+    }
+    st->probe_charge_total += dq;
+    st->cur ^= 1;
+    st->step += 1;
+    if (advance_t) st->t += R.dt;
+}
+
+__global__ __launch_bounds__(kBlock) void k_init_finish(const RunDev *rds, const int *start,
+                                                        int nruns)
+{
+    RUN_PROLOGUE();
+    if (threadIdx.x != 0) return;
+    st->cur = 0;
+    for (int s = 0; s < 2; s++) {
+        st->N[s] = R.n_init;
+        st->N_new[s] = R.n_init;
+        st->n_in[s] = 0;
+        st->far_n[s] = 0;
+        st->probe[s] = st->top[s] = 0;
+        st->last_probe[s] = st->last_top[s] = st->last_injected[s] = 0;
+        st->total_probe[s] = st->total_top[s] = st->total_injected[s] = 0;
+        st->ninj[s] = 0;
+        st->ninj_first[s] = 0;
+        st->tin[s] = 0.0;
+    }
+    st->tin_started = 0;
+    st->probe_charge_total = 0.0;
+    st->sweeps_total = 0;
+    st->solves = 0;
+    st->step = 0;
+    st->t = 0.0;
+    st->err = 0;
+}
+
+// ---------------------------------------------------------------- launchers
+
+#define LAUNCH_K(kern, L, ...)                                                         \
+    do {                                                                               \
+        if ((L).nblocks > 0)                                                           \
+            hipLaunchKernelGGL(kern, dim3((L).nblocks), dim3(kBlock), 0, (L).stream,   \
+                               (L).rds, (L).start, (L).nruns, ##__VA_ARGS__);          \
+    } while (0)
+
+void launch_clock(const Launch &L, int use_host_t, double t_host)
+{
+    LAUNCH_K(k_clock, L, use_host_t, t_host);
+}
+void launch_inject(const Launch &L) { LAUNCH_K(k_inject, L); }
+void launch_far_rank(const Launch &L) { LAUNCH_K(k_far_rank, L); }
+void launch_scan_a(const Launch &L) { LAUNCH_K(k_scan_a, L); }
+void launch_scan_b(const Launch &L) { LAUNCH_K(k_scan_b, L); }
+void launch_scan_c(const Launch &L) { LAUNCH_K(k_scan_c, L); }
+void launch_far_scatter(const Launch &L) { LAUNCH_K(k_far_scatter, L); }
+void launch_finish(const Launch &L, int advance_t) { LAUNCH_K(k_finish, L, advance_t); }
+void launch_init_finish(const Launch &L) { LAUNCH_K(k_init_finish, L); }
+
+}  // namespace cupic

@@ -1,0 +1,454 @@
+// k_particles.hip -- particle kernels.
+//
+// The push is evaluated twice per step from the same FP32 inputs, once to
+// count and once to scatter, which makes the stable cell-key counting sort
+// cost one extra read of the particle state instead of a key/permutation
+// round trip:
+//
+//   pass 1 (k_push_rank): push, wrap, absorb, classify each particle by the
+//     direction it moved (9 neighbour cells, far, probe, top) and rank it
+//     within (old cell, direction); write per-cell direction counts.  Each
+//     wave owns a strip of whole old cells, so ranks are deterministic.
+//   scan (k_cc.hip): new-cell offsets and per-slot bases.
+//   pass 2 (k_scatter): the identical push and ranks, then a store of the
+//     new state at its sorted position and the int64 fixed-point CIC charge
+//     of the new positions (the next step's deposition), reduced per wave
+//     by cell in registers and per wave-strip in LDS before HBM atomics.
+//
+// Order within a new cell: by source slot (the 3x3 neighbourhood of old
+// cells in row-major order, then far movers and injected particles), and by
+// input order within a slot -- a stable counting sort by (cell, slot).
+#include "cupic_launch.h"
+
+namespace cupic {
+
+#define RUN_PROLOGUE()                                              \
+    const int b_ = blockIdx.x;                                      \
+    const int r_ = find_run(start, nruns, b_);                      \
+    const RunDev &R = rds[r_];                                      \
+    const int local = b_ - start[r_];                               \
+    RunState *st = R.st;                                            \
+    (void)local; (void)st;
+
+constexpr double kPi = 3.1415926535897932;  // src/stdh.h:26
+
+// ---------------------------------------------------------------- init
+
+// Initial load (src/init.cu:136, 159-179): per_row particles per y row,
+// x ~ U(0,1) Lx, y = row ds + U(0,1) ds, v ~ N(0, sqrt(kT/m)).  FP64 draws
+// from Philox (counter = particle index, species, purpose 0), rounded to
+// FP32: x wrapped into [0, Lx), y moved by whole ulps into its own row.
+__global__ __launch_bounds__(kBlock) void k_create(const RunDev *rds, const int *start,
+                                                   int nruns)
+{
+    RUN_PROLOGUE();
+    int bps = (R.n_init + kBlock - 1) / kBlock;
+    int s = local / bps;
+    int g = (local - s * bps) * kBlock + threadIdx.x;
+    if (s > 1 || g >= R.n_init) return;
+    int row = g / R.per_row;
+    unsigned long long r[4];
+    draw4(R, s, 0, (unsigned long long)g, r);
+    double sigma = R.sp[s].sigma;
+    double u0 = u01_open(r[0]), u1 = u01_open(r[1]);
+    double u2 = u01_open(r[2]), u3 = u01_open(r[3]);
+    double rad = sqrt(-2.0 * log(u2));
+    double th = 2.0 * kPi * u3;
+    double x = u0 * R.Lx;
+    double y = (double)row * R.ds + u1 * R.ds;
+    double vx = sigma * (rad * cos(th));
+    double vy = sigma * (rad * sin(th));
+    float xf = (float)x;
+    if (xf < 0.0f) xf = xf + R.Lx_f;
+    if (xf >= R.Lx_f) xf = xf - R.Lx_f;
+    float yf = (float)y;
+    for (int q = 0; q < 64 && floorf(yf * R.inv_ds) > (float)row; q++) yf = nextafterf(yf, -INFINITY);
+    for (int q = 0; q < 64 && floorf(yf * R.inv_ds) < (float)row; q++) yf = nextafterf(yf, INFINITY);
+    R.px[s][1][g] = xf;
+    R.py[s][1][g] = yf;
+    R.pvx[s][1][g] = (float)vx;
+    R.pvy[s][1][g] = (float)vy;
+}
+
+constexpr int kInitMaxNcx = 4096;
+
+// Stable sort of each generated row by cell column (buffer 1 -> buffer 0)
+// and the row's cell offsets.  One workgroup per (species, row); ranks
+// within a 256-particle chunk are counted directly in LDS.  Init only.
+__global__ __launch_bounds__(kBlock) void k_init_sort(const RunDev *rds, const int *start,
+                                                      int nruns)
+{
+    RUN_PROLOGUE();
+    __shared__ int hist[kInitMaxNcx];
+    __shared__ int runc[kInitMaxNcx];
+    __shared__ int chunk[kBlock];
+    __shared__ int part[kBlock];
+    int s = local / R.ncy;
+    int row = local - s * R.ncy;
+    int ncx = R.ncx;
+    int pr = R.per_row;
+    int first = row * pr;
+    const float *X = R.px[s][1], *Y = R.py[s][1], *VX = R.pvx[s][1], *VY = R.pvy[s][1];
+    float *OX = R.px[s][0], *OY = R.py[s][0], *OVX = R.pvx[s][0], *OVY = R.pvy[s][0];
+    for (int c = threadIdx.x; c < ncx; c += kBlock) { hist[c] = 0; runc[c] = 0; }
+    __syncthreads();
+    for (int q = threadIdx.x; q < pr; q += kBlock) {
+        float fx, fy;
+        int ic, jc;
+        cell_of(R, X[first + q], Y[first + q], fx, fy, ic, jc);
+        atomicAdd(&hist[ic], 1);
+    }
+    __syncthreads();
+    // exclusive scan of hist (ncx entries) -> column starts
+    int per = (ncx + kBlock - 1) / kBlock;
+    int lo = threadIdx.x * per, hi = min(lo + per, ncx);
+    int sum = 0;
+    for (int c = lo; c < hi; c++) sum += hist[c];
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int t = 0; t < kBlock; t++) { int v = part[t]; part[t] = acc; acc += v; }
+    }
+    __syncthreads();
+    int acc = part[threadIdx.x];
+    for (int c = lo; c < hi; c++) { int v = hist[c]; hist[c] = acc; acc += v; }
+    __syncthreads();
+    int *off = R.off[s][0];
+    for (int c = threadIdx.x; c < ncx; c += kBlock) off[row * ncx + c] = first + hist[c];
+    if (row == R.ncy - 1 && threadIdx.x == 0) off[R.ncells] = R.n_init;
+    for (int base = 0; base < pr; base += kBlock) {
+        int q = base + threadIdx.x;
+        bool valid = q < pr;
+        int ic = -1;
+        float fx, fy;
+        int jc;
+        if (valid) cell_of(R, X[first + q], Y[first + q], fx, fy, ic, jc);
+        chunk[threadIdx.x] = ic;
+        __syncthreads();
+        int rank = 0, after = 0;
+        if (valid) {
+            for (int t = 0; t < (int)threadIdx.x; t++) rank += (chunk[t] == ic);
+            for (int t = threadIdx.x + 1; t < kBlock; t++) after += (chunk[t] == ic);
+            int dest = first + hist[ic] + runc[ic] + rank;
+            OX[dest] = X[first + q];
+            OY[dest] = Y[first + q];
+            OVX[dest] = VX[first + q];
+            OVY[dest] = VY[first + q];
+        }
+        __syncthreads();
+        if (valid && after == 0) runc[ic] += rank + 1;
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------- deposit tile
+
+// Adds the fixed-point CIC charge of the wave's depositing lanes to a
+// wave-private LDS node tile.  Lanes are grouped by tile cell; each group
+// reduces (sum wx, sum wy, count) and sum wx*wy, from which the four corner
+// sums follow exactly:
+//   c00 = 2^32 n - 2^16 (Swx + Swy) + Swxwy, c10 = 2^16 Swx - Swxwy,
+//   c01 = 2^16 Swy - Swxwy,                 c11 = Swxwy.
+__device__ __forceinline__ void wave_deposit(unsigned long long *tile, int TW, int dkey,
+                                             unsigned wx, unsigned wy, long long sgn, int lane)
+{
+    unsigned long long v1 = 0, v2 = 0;
+    if (dkey >= 0) {
+        v1 = (unsigned long long)wx | ((unsigned long long)wy << 23) | (1ull << 46);
+        v2 = (unsigned long long)wx * (unsigned long long)wy;
+    }
+    unsigned long long pending = __ballot(dkey >= 0);
+    while (pending) {
+        int leader = __ffsll((long long)pending) - 1;
+        int lk = __shfl(dkey, leader, kWave);
+        bool mine = (dkey == lk);
+        unsigned long long m = __ballot(mine);
+        unsigned long long a1 = wave_sum_u64(mine ? v1 : 0ull);
+        unsigned long long a2 = wave_sum_u64(mine ? v2 : 0ull);
+        if (lane == leader) {
+            long long sx = (long long)(a1 & ((1ull << 23) - 1));
+            long long sy = (long long)((a1 >> 23) & ((1ull << 23) - 1));
+            long long n = (long long)(a1 >> 46);
+            long long sxy = (long long)a2;
+            long long c11 = sxy;
+            long long c10 = (sx << 16) - sxy;
+            long long c01 = (sy << 16) - sxy;
+            long long c00 = (n << 32) - ((sx + sy) << 16) + sxy;
+            atomicAdd(&tile[lk], (unsigned long long)(sgn * c00));
+            atomicAdd(&tile[lk + 1], (unsigned long long)(sgn * c10));
+            atomicAdd(&tile[lk + TW], (unsigned long long)(sgn * c01));
+            atomicAdd(&tile[lk + TW + 1], (unsigned long long)(sgn * c11));
+        }
+        pending &= ~m;
+    }
+}
+
+// Tile rows j0-1 .. j0+2, columns col0-1 .. col0+nc+1 (wrapped) -> HBM.
+__device__ __forceinline__ void flush_tile(const RunDev &R, const unsigned long long *tile,
+                                           int TW, int j0, int col0, int lane)
+{
+    for (int q = lane; q < 4 * TW; q += kWave) {
+        unsigned long long v = tile[q];
+        if (v == 0ull) continue;
+        int row = j0 - 1 + q / TW;
+        int col = col0 - 1 + (q % TW);
+        if (col < 0) col += R.ncx;
+        if (col >= R.ncx) col -= R.ncx;
+        if (row < 0 || row > R.ncy) continue;
+        atomicAdd(&R.acc[row * R.ncx + col], v);
+    }
+}
+
+struct TaskInfo {
+    int s, j0, col0, nc;
+    bool ok;
+};
+
+__device__ __forceinline__ TaskInfo task_of(const RunDev &R, int local)
+{
+    TaskInfo t;
+    int task = local * (kBlock / kWave) + (threadIdx.x >> 6);
+    int per_species = R.ncy * R.tpr;
+    t.ok = task < 2 * per_species;
+    t.s = task / per_species;
+    int t2 = task - t.s * per_species;
+    t.j0 = t2 / R.tpr;
+    t.col0 = (t2 - t.j0 * R.tpr) * R.C;
+    t.nc = min(R.C, R.ncx - t.col0);
+    return t;
+}
+
+// Standalone deposition of the current (sorted) state: charge_deposition
+// (src/mesh.cu:15-51) in int64 fixed point.
+__global__ __launch_bounds__(kBlock) void k_deposit(const RunDev *rds, const int *start,
+                                                    int nruns)
+{
+    RUN_PROLOGUE();
+    __shared__ unsigned long long tiles[kBlock / kWave][4 * kTileW];
+    TaskInfo T = task_of(R, local);
+    if (!T.ok) return;
+    int lane = threadIdx.x & (kWave - 1);
+    unsigned long long *tile = tiles[threadIdx.x >> 6];
+    int TW = T.nc + 3;
+    for (int q = lane; q < 4 * TW; q += kWave) tile[q] = 0ull;
+    wave_lds_sync();
+    int cur = st->cur;
+    const int *off = R.off[T.s][cur];
+    int c0 = T.j0 * R.ncx + T.col0;
+    int p0 = off[c0], p1 = off[c0 + T.nc];
+    const float *X = R.px[T.s][cur], *Y = R.py[T.s][cur];
+    long long sgn = T.s == 0 ? -1 : 1;
+    for (int i0 = p0; i0 < p1; i0 += kWave) {
+        int i = i0 + lane;
+        int dkey = -1;
+        unsigned wx = 0, wy = 0;
+        if (i < p1) {
+            float fx, fy;
+            int ic, jc;
+            cell_of(R, X[i], Y[i], fx, fy, ic, jc);
+            int lc = ic - T.col0 + 1, lr = jc - (T.j0 - 1);
+            fixed_weights(fx, fy, wx, wy);
+            if (lr >= 0 && lr <= 2 && lc >= 0 && lc <= T.nc + 1) {
+                dkey = lr * TW + lc;
+            } else {  // outside the strip: straight to HBM
+                int i1 = ic + 1 == R.ncx ? 0 : ic + 1;
+                long long gx = 65536 - (long long)wx, gy = 65536 - (long long)wy;
+                atomicAdd(&R.acc[jc * R.ncx + ic], (unsigned long long)(sgn * gx * gy));
+                atomicAdd(&R.acc[jc * R.ncx + i1], (unsigned long long)(sgn * (long long)wx * gy));
+                atomicAdd(&R.acc[(jc + 1) * R.ncx + ic], (unsigned long long)(sgn * gx * (long long)wy));
+                atomicAdd(&R.acc[(jc + 1) * R.ncx + i1],
+                          (unsigned long long)(sgn * (long long)wx * (long long)wy));
+            }
+        }
+        wave_deposit(tile, TW, dkey, wx, wy, sgn, lane);
+    }
+    wave_lds_sync();
+    flush_tile(R, tile, TW, T.j0, T.col0, lane);
+}
+
+// Initial velocity stagger, src/init.cu:793-833: v -= dt/2 * qE/m.
+__global__ __launch_bounds__(kBlock) void k_half_kick(const RunDev *rds, const int *start,
+                                                      int nruns)
+{
+    RUN_PROLOGUE();
+    int bps = (R.n_init + kBlock - 1) / kBlock;
+    int s = local / bps;
+    if (s > 1) return;
+    int cur = st->cur;
+    int n = st->N[s];
+    float hk = R.sp[s].hkick;
+    const float *X = R.px[s][cur], *Y = R.py[s][cur];
+    float *VX = R.pvx[s][cur], *VY = R.pvy[s][cur];
+    for (int i = (local - s * bps) * kBlock + threadIdx.x; i < n; i += bps * kBlock) {
+        float fx, fy, ex, ey;
+        int ic, jc;
+        cell_of(R, X[i], Y[i], fx, fy, ic, jc);
+        gather(R, R.E, fx, fy, ic, jc, ex, ey);
+        VX[i] = VX[i] + hk * ex;
+        VY[i] = VY[i] + hk * ey;
+    }
+}
+
+// ---------------------------------------------------------------- push passes
+
+template <bool kScatter>
+__device__ __forceinline__ void push_task(const RunDev &R, RunState *st, const TaskInfo &T,
+                                          int *cntw, unsigned long long *tile, int lane)
+{
+    const int s = T.s, j0 = T.j0, col0 = T.col0, nc = T.nc;
+    const int cur = st->cur;
+    const int *off = R.off[s][cur];
+    const int c0 = j0 * R.ncx + col0;
+    const int p0 = off[c0], p1 = off[c0 + nc];
+    const float *X = R.px[s][cur], *Y = R.py[s][cur];
+    const float *VX = R.pvx[s][cur], *VY = R.pvy[s][cur];
+    float *OX = R.px[s][cur ^ 1], *OY = R.py[s][cur ^ 1];
+    float *OVX = R.pvx[s][cur ^ 1], *OVY = R.pvy[s][cur ^ 1];
+    const int *base = R.base[s];
+    const float2 *E = R.E;
+    const float qdtm = R.sp[s].qdtm;
+    const long long sgn = s == 0 ? -1 : 1;
+    const int TW = nc + 3;
+    for (int q = lane; q < nc * 9; q += kWave) cntw[q] = 0;
+    if (kScatter)
+        for (int q = lane; q < 4 * TW; q += kWave) tile[q] = 0ull;
+    wave_lds_sync();
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    const unsigned long long upto = lt | (1ull << lane);
+    unsigned long long probe_n = 0, top_n = 0;
+    bool bad = false;
+    for (int i0 = p0; i0 < p1; i0 += kWave) {
+        const int i = i0 + lane;
+        const bool valid = i < p1;
+        float x = 0.0f, y = 0.0f, vx = 0.0f, vy = 0.0f;
+        if (valid) { x = X[i]; y = Y[i]; vx = VX[i]; vy = VY[i]; }
+        Pushed o = push_one(R, E, qdtm, x, y, vx, vy);
+        int dir = valid ? o.dir : (int)kDirNone;
+        int cl = o.oic - col0;
+        if (dir < 9 && (o.ojc != j0 || cl < 0 || cl >= nc)) {  // not in this strip
+            dir = kDirFar;
+            bad = true;
+        }
+        // rank within (old cell, direction)
+        unsigned long long mymask = 0;
+#pragma unroll
+        for (int dd = 0; dd < 9; dd++) {
+            unsigned long long m = __ballot(dir == dd);
+            if (dir == dd) mymask = m;
+        }
+        int clp = __shfl_up(cl, 1, kWave);
+        bool head = valid && (lane == 0 || cl != clp);
+        unsigned long long heads = __ballot(head);
+        int rank = 0;
+        bool last = false;
+        if (dir < 9) {
+            int seg_start = 63 - __clzll((long long)(heads & upto));
+            unsigned long long after = heads & ~upto;
+            unsigned long long hi = after ? ((1ull << (__ffsll((long long)after) - 1)) - 1ull) : ~0ull;
+            unsigned long long seg = hi & ~((1ull << seg_start) - 1ull);
+            unsigned long long grp = mymask & seg;
+            rank = cntw[cl * 9 + dir] + __popcll(grp & lt);
+            last = (grp & ~upto) == 0ull;
+        }
+        wave_lds_sync();
+        if (dir < 9 && last) cntw[cl * 9 + dir] = rank + 1;
+        wave_lds_sync();
+        if (!kScatter) {
+            unsigned long long fm = __ballot(dir == kDirFar);
+            if (fm) {
+                int leader = __ffsll((long long)fm) - 1;
+                int fb = 0;
+                if (lane == leader) fb = atomicAdd(&st->far_n[s], __popcll(fm));
+                fb = __shfl(fb, leader, kWave);
+                if (dir == kDirFar) {
+                    int slot = fb + __popcll(fm & lt);
+                    if (slot < R.far_cap) {
+                        FarRec rec;
+                        rec.key = o.njc * R.ncx + o.nic;
+                        rec.idx = i;
+                        rec.rank = 0;
+                        rec.pad = 0;
+                        rec.x = o.x; rec.y = o.y; rec.vx = o.vx; rec.vy = o.vy;
+                        R.far[s][slot] = rec;
+                    } else {
+                        atomicOr(&st->err, kErrFarOverflow);
+                    }
+                }
+            }
+            probe_n += __popcll(__ballot(dir == kDirProbe));
+            top_n += __popcll(__ballot(dir == kDirTop));
+        } else {
+            int dkey = -1;
+            unsigned wx = 0, wy = 0;
+            if (dir < 9) {
+                int key = o.njc * R.ncx + o.nic;
+                int dest = base[key * 10 + (8 - dir)] + rank;
+                if ((unsigned)dest < (unsigned)R.cap) {
+                    OX[dest] = o.x;
+                    OY[dest] = o.y;
+                    OVX[dest] = o.vx;
+                    OVY[dest] = o.vy;
+                } else {
+                    bad = true;
+                }
+                fixed_weights(o.fx, o.fy, wx, wy);
+                dkey = (o.njc - (j0 - 1)) * TW + (cl + o.di + 1);
+            }
+            wave_deposit(tile, TW, dkey, wx, wy, sgn, lane);
+        }
+    }
+    wave_lds_sync();
+    if (!kScatter) {
+        int *cnt = R.cnt[s] + (size_t)c0 * 9;
+        for (int q = lane; q < nc * 9; q += kWave) cnt[q] = cntw[q];
+        if (lane == 0) {
+            if (probe_n) atomicAdd(&st->probe[s], probe_n);
+            if (top_n) atomicAdd(&st->top[s], top_n);
+        }
+        if (__ballot(bad) && lane == 0) atomicOr(&st->err, kErrSortInvariant);
+    } else {
+        flush_tile(R, tile, TW, j0, col0, lane);
+        if (__ballot(bad) && lane == 0) atomicOr(&st->err, kErrSortInvariant);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_push_rank(const RunDev *rds, const int *start,
+                                                      int nruns)
+{
+    RUN_PROLOGUE();
+    __shared__ int cnts[kBlock / kWave][kCmax * 9];
+    TaskInfo T = task_of(R, local);
+    if (!T.ok) return;
+    push_task<false>(R, st, T, cnts[threadIdx.x >> 6], nullptr, threadIdx.x & (kWave - 1));
+}
+
+__global__ __launch_bounds__(kBlock) void k_scatter(const RunDev *rds, const int *start,
+                                                    int nruns)
+{
+    RUN_PROLOGUE();
+    __shared__ int cnts[kBlock / kWave][kCmax * 9];
+    __shared__ unsigned long long tiles[kBlock / kWave][4 * kTileW];
+    TaskInfo T = task_of(R, local);
+    if (!T.ok) return;
+    push_task<true>(R, st, T, cnts[threadIdx.x >> 6], tiles[threadIdx.x >> 6],
+                    threadIdx.x & (kWave - 1));
+}
+
+// ---------------------------------------------------------------- launchers
+
+#define LAUNCH_K(kern, L, ...)                                                         \
+    do {                                                                               \
+        if ((L).nblocks > 0)                                                           \
+            hipLaunchKernelGGL(kern, dim3((L).nblocks), dim3(kBlock), 0, (L).stream,   \
+                               (L).rds, (L).start, (L).nruns, ##__VA_ARGS__);          \
+    } while (0)
+
+void launch_create(const Launch &L) { LAUNCH_K(k_create, L); }
+void launch_init_sort(const Launch &L) { LAUNCH_K(k_init_sort, L); }
+void launch_deposit(const Launch &L) { LAUNCH_K(k_deposit, L); }
+void launch_half_kick(const Launch &L) { LAUNCH_K(k_half_kick, L); }
+void launch_push_rank(const Launch &L) { LAUNCH_K(k_push_rank, L); }
+void launch_scatter(const Launch &L) { LAUNCH_K(k_scatter, L); }
+
+}  // namespace cupic

@@ -1,0 +1,199 @@
+/*
+ * cupic.h -- C-ABI of the MI355X-native electrostatic PIC cycle.
+ *
+ * Drop-in boundary for cupic's module entry points (init / mesh / particles /
+ * cc / diagnostic).  Every function returns an int status (CUPIC_OK == 0,
+ * negative on error) and never exits the process; cupic_last_error() gives
+ * the message.  Parameters travel in an explicit cupic_params struct instead
+ * of the reference's memoised global getters (src/init.h:40-63), so one
+ * process can host many runs.
+ *
+ * Units are normalised as in the reference (src/init.cu:451-789): lengths in
+ * Debye lengths, time in 1/omega_pe, mass in electron masses, potential in
+ * kTe/e.  Grids are exported in the reference layout: nny rows of nnx nodes,
+ * index i + j*nnx, column nnx-1 being the periodic image of column 0.
+ *
+ * Threading: one host thread per run at a time.  Each run owns a HIP stream;
+ * calls enqueue work and return, except those that return host values.
+ */
+#ifndef CUPIC_H
+#define CUPIC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CUPIC_OK 0
+#define CUPIC_ERR_ARG (-1)        /* invalid argument or parameter           */
+#define CUPIC_ERR_HIP (-2)        /* HIP runtime error                       */
+#define CUPIC_ERR_ALLOC (-3)      /* device allocation failed or over budget */
+#define CUPIC_ERR_CAPACITY (-4)   /* particle / injection / far-list overflow */
+#define CUPIC_ERR_STATE (-5)      /* call out of order (e.g. cc before mover) */
+#define CUPIC_ERR_NUMERIC (-6)    /* non-finite values in the particle state  */
+#define CUPIC_ERR_IO (-7)         /* file could not be read or written        */
+
+/* Poisson stopping rules */
+#define CUPIC_JACOBI_REFERENCE 0  /* max|dphi| < tol after >= min sweeps (src/mesh.cu:94-114) */
+#define CUPIC_JACOBI_RESIDUAL 1   /* FP32 sweeps, FP64 residual checked every check_every     */
+#define CUPIC_JACOBI_REFINE 2     /* FP64 residual + FP32 correction solve (refinement)        */
+
+/* quirk flags */
+#define CUPIC_QUIRK_Q7_REFERENCE_KICK 1u /* injection half-kick sign as committed (src/cc.cu:243-244) */
+
+typedef struct cupic_params {
+    int32_t ncx, ncy;        /* cells (nnx = ncx+1, nny = ncy+1 nodes)           */
+    double ds, dt;           /* cell size, time step                             */
+    double n;                /* density, particles per Debye volume; ppc = n ds^3 */
+    double mi, kti;          /* ion mass / m_e, ion temperature / kTe            */
+    double me, kte;          /* 1 and 1 (src/init.cu:487-494, 512-519)           */
+    double phi_p;            /* probe bias, kTe/e                                */
+    double eps0;             /* permittivity; 0 selects the normalised value n   */
+    uint64_t seed, run_id;   /* RNG key                                          */
+    uint32_t quirks;
+    int32_t jacobi_mode;     /* CUPIC_JACOBI_*                                   */
+    int32_t jacobi_min_sweeps;   /* mode 0 minimum (reference: max(nnx,nny)+1)   */
+    int32_t jacobi_max_sweeps;   /* cap on sweeps per solve                       */
+    int32_t jacobi_check_every;  /* modes 1, 2: sweeps between FP64 residuals    */
+    int32_t pad0;
+    double jacobi_tol;       /* per-step tolerance (reference 1e-4, src/main.cu:64) */
+    double jacobi_tol_init;  /* init tolerance (reference 1e-3, src/init.cu:254)     */
+    /* derived by cupic_params_derive */
+    double Lx, Ly;
+    double dtin_e, dtin_i;   /* injection intervals (src/init.cu:686-714)        */
+    int64_t per_row;         /* initial particles per y row (src/init.cu:136)    */
+} cupic_params;
+
+typedef struct cupic_run cupic_run;
+typedef struct cupic_batch cupic_batch;
+
+/* ---- device and parameters ------------------------------------------ */
+
+/* Replaces init_dev (src/init.cu:15-55): select a device, report it. */
+int cupic_init_dev(int device, char *name, int name_len, int *compute_units);
+
+/* Config-1 defaults: 128x64 cells, ds 0.5, dt 0.05, 20 ppc, mi 100, phi_p -5. */
+int cupic_params_default(cupic_params *p);
+
+/* Replaces the init_* getters (src/init.cu:451-789): fills the derived
+ * fields and checks the parameters. */
+int cupic_params_derive(cupic_params *p);
+
+/* Reads the reference's positional input file (src/init.cu:424-447,
+ * input/input_data) and converts its SI values (src/init.cu:523-556,
+ * 612-630, 718-733) into normalised parameters. */
+int cupic_params_from_input_file(const char *path, cupic_params *p, int32_t *n_ini,
+                                 int32_t *n_prev, int32_t *n_save, int32_t *n_fin);
+
+/* ---- run lifetime ---------------------------------------------------- */
+
+/* Allocates every device buffer of one run (the reference's main owns them,
+ * src/main.cu:42-44, and init_sim allocates, src/init.cu:147-156, 225-232).
+ * capacity_hint: particles per species; 0 picks N + headroom. */
+int cupic_run_create(const cupic_params *p, int64_t capacity_hint, cupic_run **out);
+int cupic_run_destroy(cupic_run *r);
+const char *cupic_last_error(const cupic_run *r);   /* r may be NULL */
+int cupic_synchronize(cupic_run *r);
+/* device bytes this run holds */
+int cupic_run_footprint(const cupic_params *p, int64_t capacity_hint, int64_t *bytes);
+
+/* ---- init (src/init.cu) ------------------------------------------------ */
+
+/* init_sim with n_ini == 0 (src/init.cu:57-83): create_particles,
+ * initialize_mesh, adjust_leap_frog.  *t is set to 0. */
+int cupic_init_sim(cupic_run *r, double *t);
+/* create_particles (src/init.cu:106-198): device Philox draws, cell-sorted */
+int cupic_create_particles(cupic_run *r);
+/* initialize_mesh (src/init.cu:202-260): linear phi, deposit, solve, field */
+int cupic_initialize_mesh(cupic_run *r);
+/* adjust_leap_frog (src/init.cu:264-323): v -= dt/2 * qE/m */
+int cupic_adjust_leap_frog(cupic_run *r);
+
+/* ---- mesh (src/mesh.h:27-29) ----------------------------------------- */
+
+/* charge_deposition (src/mesh.cu:15-51): int64 fixed-point CIC charge of
+ * both species, converted to rho. */
+int cupic_charge_deposition(cupic_run *r);
+/* poisson_solver (src/mesh.cu:55-120).  tol < 0 uses the run's per-step
+ * tolerance.  Returns sweeps run and the final residual (max |update|). */
+int cupic_poisson_solver(cupic_run *r, double tol, int32_t *sweeps, double *resid);
+/* field_solver (src/mesh.cu:124-154): E = -grad phi */
+int cupic_field_solver(cupic_run *r);
+
+/* ---- particles (src/particles.h:28) ----------------------------------- */
+
+/* particle_mover (src/particles.cu:15-102): CIC gather + leapfrog of both
+ * species.  The pushed state is committed by the following cupic_cc. */
+int cupic_particle_mover(cupic_run *r);
+
+/* ---- boundary conditions (src/cc.h:29-32) ------------------------------ */
+
+/* cc (src/cc.cu:15-69): probe / top absorption with probe-current tally,
+ * flux-Maxwellian injection at y = Ly, periodic x, cell-key re-sort. */
+int cupic_cc(cupic_run *r, double t);
+
+/* one full cycle, src/main.cu:61-73: deposit, solve, field, move, cc(t) */
+int cupic_step(cupic_run *r, double t);
+
+/* ---- diagnostics (src/diagnostic.h:27-31) ------------------------------ */
+
+/* number_of_particles (src/diagnostic.cu:15-44); species 0 = e-, 1 = ions */
+int cupic_number_of_particles(cupic_run *r, int species, int64_t *n);
+/* probe tally of the last cc and running totals (no reference counterpart:
+ * src/cc.cu:184 computes the probe count and discards it) */
+int cupic_probe_tally(cupic_run *r, int64_t last_probe[2], int64_t last_top[2],
+                      int64_t last_injected[2], int64_t total_probe[2],
+                      double *probe_charge_total);
+int cupic_last_solve(cupic_run *r, int32_t *sweeps, double *resid);
+/* Jacobi sweeps and solves since init_sim */
+int cupic_solver_totals(cupic_run *r, int64_t *sweeps_total, int64_t *solves);
+int cupic_get_particles(cupic_run *r, int species, float *x, float *y, float *vx, float *vy,
+                        int64_t capacity, int64_t *n);
+/* uploads a particle state (any order); it is cell-sorted on the device */
+int cupic_set_particles(cupic_run *r, int species, const float *x, const float *y,
+                        const float *vx, const float *vy, int64_t n);
+/* cell offsets of the sorted state, ncx*ncy + 1 entries */
+int cupic_get_offsets(cupic_run *r, int species, int32_t *offsets);
+
+#define CUPIC_MESH_RHO 0     /* float, charge density                      */
+#define CUPIC_MESH_PHI 1     /* double, potential                          */
+#define CUPIC_MESH_EX 2      /* float                                      */
+#define CUPIC_MESH_EY 3      /* float                                      */
+#define CUPIC_MESH_CHARGE 4  /* int64, fixed-point node charge, 2^-32 e    */
+/* copies a mesh quantity into host memory in the reference layout (nnx*nny) */
+int cupic_get_mesh(cupic_run *r, int which, void *host, int64_t count);
+/* sets the potential from a host array in the reference layout (doubles) */
+int cupic_set_phi(cupic_run *r, const double *phi, int64_t count);
+int cupic_get_clock(cupic_run *r, double tin[2], uint64_t injected_total[2], int32_t *started);
+int cupic_set_clock(cupic_run *r, const double tin[2], const uint64_t injected_total[2],
+                    int32_t started);
+
+/* snapshot writers in the reference's text formats (src/diagnostic.cu:48-133) */
+int cupic_particles_snapshot(cupic_run *r, int species, const char *path, double t);
+int cupic_mesh_snapshot(cupic_run *r, int which, const char *path);
+
+/* ---- batched runs -------------------------------------------------------- */
+
+/* Groups runs (same Jacobi mode and check interval) so that one launch of
+ * each kernel advances all of them; the step is captured once in a HIP graph
+ * and replayed.  The runs must have been initialised. */
+int cupic_batch_create(cupic_run *const *runs, int32_t nruns, int32_t use_graph,
+                       cupic_batch **out);
+int cupic_batch_step(cupic_batch *b, int32_t nsteps);
+int cupic_batch_destroy(cupic_batch *b);
+const char *cupic_batch_last_error(const cupic_batch *b);
+
+/* ---- timing ------------------------------------------------------------- */
+
+/* Per-kernel-class device time (ms) accumulated with HIP events on the run's
+ * stream while timing is on; classes are listed by cupic_timing_names. */
+int cupic_timing_enable(cupic_run *r, int32_t on);
+int cupic_timing_read(cupic_run *r, double *ms, int64_t *launches, int32_t n);
+const char *cupic_timing_names(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

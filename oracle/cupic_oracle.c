@@ -759,8 +759,8 @@ static void round_initial(const orc_f32c *c, int64_t row, double x, double y, do
     if (xf < 0.0f) xf += c->Lx;
     if (xf >= c->Lx) xf -= c->Lx;
     float yf = (float)y;
-    while (floorf(yf * c->inv_ds) > (float)row) yf = nextafterf(yf, -INFINITY);
-    while (floorf(yf * c->inv_ds) < (float)row) yf = nextafterf(yf, INFINITY);
+    for (int q = 0; q < 64 && floorf(yf * c->inv_ds) > (float)row; q++) yf = nextafterf(yf, -INFINITY);
+    for (int q = 0; q < 64 && floorf(yf * c->inv_ds) < (float)row; q++) yf = nextafterf(yf, INFINITY);
     *ox = xf; *oy = yf; *ovx = (float)vx; *ovy = (float)vy;
 }
 

@@ -1,0 +1,244 @@
+"""GPU parity: the HIP path (through the C-ABI) against the CPU oracle.
+
+Bit-exact (integer, index and FP32-state work, against the oracle's FP32
+mirror of the same operations): initial load and cell sort, offsets, the
+half-step stagger, one full push/absorb/inject/sort step, probe tallies and
+the int64 fixed-point charge.  Within stated tolerance (against the FP64
+layer): Jacobi potential after the same sweeps, convergence to the FP64
+residual tolerance, the field.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import helpers as H
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+cp = pytest.importorskip("cupic_amd")
+
+
+def small(**kw):
+    base = dict(ncx=64, ncy=32, ppc=10, phi_p=-5.0, jacobi_max_sweeps=200)
+    base.update(kw)
+    return cp.params(**base)
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def dev():
+    name, cus = cp.init_dev(0)
+    say("device:", name, cus, "CUs")
+    return name
+
+
+@pytest.mark.parametrize("shape", [(64, 32, 10), (37, 9, 7), (128, 64, 20)])
+def test_init_load_and_sort_bit_exact(dev, shape):
+    ncx, ncy, ppc = shape
+    p = small(ncx=ncx, ncy=ncy, ppc=ppc)
+    say("init sort", shape)
+    run = cp.Run(p)
+    run.create_particles()
+    op = H.oracle_params(p)
+    for s in (0, 1):
+        got = run.particles(s)
+        off = run.offsets(s)
+        want, woff = H.mirror_init(op, s)
+        assert np.array_equal(off, woff.astype(np.int32))
+        for g, w in zip(got, want):
+            assert np.array_equal(g.view(np.uint32), w.view(np.uint32)), H.mismatch(g, w)
+
+
+def test_half_kick_bit_exact(dev):
+    p = small()
+    run = cp.Run(p)
+    run.create_particles()
+    run.initialize_mesh()
+    E = H.interleaved_E(run)
+    run.adjust_leap_frog()
+    op = H.oracle_params(p)
+    c = O.f32c(op)
+    for s in (0, 1):
+        (x, y, vx, vy), _ = H.mirror_init(op, s)
+        O.lib().orc_mirror_half_kick(c, s, len(x), x, y, vx, vy, E)
+        got = run.particles(s)
+        assert np.array_equal(got[2], vx) and np.array_equal(got[3], vy)
+
+
+@pytest.mark.parametrize("kw", [dict(), dict(ncx=37, ncy=9, ppc=7, phi_p=2.0),
+                                dict(quirks=1, mi=4.0), dict(dt=0.2, ppc=5)])
+def test_full_step_bit_exact(dev, kw):
+    """deposit -> solve -> field -> move -> cc(t), compared with the FP32
+    mirror given the run's own field and clock: sorted states, offsets,
+    probe/top tallies and the next step's fixed-point charge."""
+    p = small(**kw)
+    say("full step", kw)
+    run = cp.Run(p)
+    run.init_sim()
+    op = H.oracle_params(p)
+    t = p.dt
+    for step in range(3):
+        pre = [run.particles(s) for s in (0, 1)]
+        run.charge_deposition()
+        run.poisson_solver()
+        run.field_solver()
+        E = H.interleaved_E(run)
+        tin, ninj, started = run.clock()
+        run.particle_mover()
+        run.cc(t)
+        tally = run.probe_tally()
+        posts = []
+        for s in (0, 1):
+            inj, tin_after = H.injection_for_step(op, s, t, tin[s], started, ninj[s], E)
+            out, off, pr, tp, nf = H.mirror_step(op, s, pre[s], E, inj)
+            got = run.particles(s)
+            assert np.array_equal(run.offsets(s), off.astype(np.int32)), (step, s)
+            for g, w in zip(got, out):
+                assert np.array_equal(g.view(np.uint32), w.view(np.uint32)), (step, s, H.mismatch(g, w))
+            assert tally["probe"][s] == pr and tally["top"][s] == tp
+            assert tally["injected"][s] == len(inj[0])
+            assert run.clock()[0][s] == tin_after
+            posts.append(out)
+        charge = H.periodic(run.mesh(cp.MESH_CHARGE))
+        assert np.array_equal(charge, H.fixed_charge(op, posts))
+        t += p.dt
+
+
+def test_charge_conservation_exact(dev):
+    p = small()
+    run = cp.Run(p)
+    run.init_sim()
+    t = p.dt
+    for _ in range(5):
+        run.step(t)
+        t += p.dt
+    ne, ni = run.number_of_particles(0), run.number_of_particles(1)
+    q = run.mesh(cp.MESH_CHARGE)[:, :-1]
+    assert int(q.sum()) == (ni - ne) * 2**32
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_jacobi_matches_fp64_sweeps(dev, mode):
+    """Same start potential, same charge, same number of sweeps: the GPU
+    potential agrees with the FP64 oracle (FP32 sweeps: 2e-6 of max|phi|;
+    refinement: 1e-8)."""
+    p = small(jacobi_mode=mode, jacobi_max_sweeps=300, jacobi_check_every=50)
+    run = cp.Run(p)
+    run.init_sim()
+    run.step(p.dt)
+    acc = H.periodic(run.mesh(cp.MESH_CHARGE)).astype(np.float64)
+    op = H.oracle_params(p)
+    rho = acc * 2.0**-32 / p.ds**3
+    rho[0] *= 2.0
+    rho[-1] *= 2.0
+    phi0 = np.zeros((p.ncy + 1, p.ncx))
+    O.lib().orc_phi_linear(p.ncx, p.ncy, p.phi_p, phi0.ravel())
+    jj, ii = np.mgrid[0:p.ncy + 1, 0:p.ncx]
+    phi0[1:-1] += 0.3 * np.sin(2 * np.pi * ii[1:-1] / p.ncx) * np.sin(np.pi * jj[1:-1] / p.ncy)
+    ref = np.concatenate([phi0, phi0[:, :1]], 1)
+    run.set_phi(ref)
+    run.charge_deposition()
+    sw, res = run.poisson_solver(tol=0.0)
+    assert sw == 300
+    got = H.periodic(run.mesh(cp.MESH_PHI))
+    want = phi0.ravel().copy()
+    import ctypes as C
+    r = C.c_double()
+    s2 = O.lib().orc_poisson_solve(p.ncx, p.ncy, p.ds, p.eps0, rho.ravel().copy(), want, mode,
+                                   0.0, 0, 300, 50, C.byref(r))
+    assert s2 == 300
+    err = H.rel_err(got, want.reshape(got.shape))
+    say("mode", mode, "phi rel err", err, "resid gpu", res, "oracle", r.value)
+    assert err < (1e-8 if mode == 2 else 2e-6)
+    if mode:
+        assert res == pytest.approx(r.value, rel=1e-3 if mode == 2 else 0.2, abs=2e-7)
+    # the field of that potential
+    run.field_solver()
+    Ex = H.periodic(run.mesh(cp.MESH_EX))
+    Ey = H.periodic(run.mesh(cp.MESH_EY))
+    fx = np.zeros(got.size)
+    fy = np.zeros(got.size)
+    O.lib().orc_field(p.ncx, p.ncy, p.ds, got.ravel().copy(), fx, fy)
+    assert H.rel_err(Ex, fx.reshape(Ex.shape)) < 1e-6
+    assert H.rel_err(Ey, fy.reshape(Ey.shape)) < 1e-6
+
+
+def test_refinement_reaches_fp64_tolerance(dev):
+    """Mode 2 drives the FP64 residual below 1e-6 where it is reachable
+    (small grid, generous cap) and reports the sweeps the oracle needs."""
+    p = small(ncx=32, ncy=16, jacobi_mode=2, jacobi_max_sweeps=20000, jacobi_tol=1e-6)
+    run = cp.Run(p)
+    run.init_sim()
+    run.step(p.dt)
+    run.charge_deposition()
+    sw, res = run.poisson_solver(tol=1e-7)
+    assert res <= 1e-7 and sw < 20000
+    acc = H.periodic(run.mesh(cp.MESH_PHI))
+    assert np.isfinite(acc).all()
+
+
+def test_injection_and_absorption_counts(dev):
+    p = small(phi_p=-5.0)
+    run = cp.Run(p)
+    run.init_sim()
+    n0 = [run.number_of_particles(s) for s in (0, 1)]
+    t = p.dt
+    for _ in range(20):
+        run.step(t)
+        tl = run.probe_tally()
+        n1 = [run.number_of_particles(s) for s in (0, 1)]
+        for s in (0, 1):
+            assert n1[s] == n0[s] - tl["probe"][s] - tl["top"][s] + tl["injected"][s]
+            off = run.offsets(s)
+            assert off[0] == 0 and off[-1] == n1[s] and np.all(np.diff(off) >= 0)
+        n0 = n1
+        t += p.dt
+
+
+def test_runs_are_deterministic(dev):
+    p = small()
+    a, b = cp.Run(p), cp.Run(p)
+    for r in (a, b):
+        r.init_sim()
+    t = p.dt
+    for _ in range(8):
+        a.step(t)
+        b.step(t)
+        t += p.dt
+    for s in (0, 1):
+        for u, v in zip(a.particles(s), b.particles(s)):
+            assert np.array_equal(u, v)
+
+
+def test_batch_graph_equals_plain_runs(dev):
+    """Alone vs in a mix: runs of different shapes advanced together in one
+    captured graph give bit-identical states to each run stepped alone."""
+    shapes = [dict(ncx=64, ncy=32, ppc=10), dict(ncx=40, ncy=20, ppc=6, phi_p=1.0, seed=7),
+              dict(ncx=96, ncy=16, ppc=12, phi_p=-8.0, seed=9)]
+    alone, mixed = [], []
+    for kw in shapes:
+        p = small(jacobi_max_sweeps=100, **kw)
+        for lst in (alone, mixed):
+            r = cp.Run(p)
+            r.init_sim()
+            lst.append(r)
+    for r in alone:
+        t = r.p.dt
+        for _ in range(6):
+            r.step(t)
+            t += r.p.dt
+    for graph in (True,):
+        b = cp.Batch(mixed, graph=graph)
+        b.step(6)
+        b.close()
+    for ra, rm in zip(alone, mixed):
+        for s in (0, 1):
+            for u, v in zip(ra.particles(s), rm.particles(s)):
+                assert np.array_equal(u, v)
+        assert np.array_equal(ra.mesh(cp.MESH_PHI), rm.mesh(cp.MESH_PHI))
