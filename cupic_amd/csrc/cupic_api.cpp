@@ -60,10 +60,13 @@ struct cupic_run {
     int *d_plan = nullptr;
     int nblk[kShapeCount] = {};
     hipStream_t stream = nullptr;
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     std::vector<void *> allocs;
     std::string err;
     int moved = 0;          // particle_mover done, cc pending
     bool acc_valid = false; // acc holds the charge of the current particles
+    bool clock_forked = false;  // the coming cc's clock already runs on the side stream
     bool initialised = false;
     double t_next = 0.0;
     int64_t footprint = 0;
@@ -81,6 +84,8 @@ struct cupic_batch {
     int *d_plan = nullptr;   // kShapeCount * (nruns + 1)
     int nblk[kShapeCount] = {};
     hipStream_t stream = nullptr;
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     bool use_graph = false;
@@ -102,6 +107,8 @@ int shape_blocks(const RunDev &h, Shape s)
     case kShapeInject: return 2 * ((h.max_inject + kBlock - 1) / kBlock);
     case kShapeRows: return 2 * h.ncy;
     case kShapeInit: return 2 * ((h.n_init + kBlock - 1) / kBlock);
+    case kShapeJacBlk:
+        return ((h.ncx + kJacTX - 1) / kJacTX) * ((h.ncy - 1 + kJacTY - 1) / kJacTY);
     default: return 0;
     }
 }
@@ -120,15 +127,34 @@ struct Target {
     int nruns;
     hipStream_t stream;
     cupic_run *timed;     // non-null: record timing events for this run
+    hipStream_t side = nullptr;   // injection clock runs here when set
+    hipEvent_t fork = nullptr, join = nullptr;
     Launch L(Shape s) const
     {
         return Launch{rds, plan + (size_t)s * (nruns + 1), nruns, nblk[s], stream};
+    }
+    Launch Lside(Shape s) const
+    {
+        return Launch{rds, plan + (size_t)s * (nruns + 1), nruns, nblk[s], side};
     }
 };
 
 Target target_of(cupic_run *r)
 {
-    return Target{r->d_rd, r->d_plan, r->nblk, 1, r->stream, r->timing ? r : nullptr};
+    Target T{r->d_rd, r->d_plan, r->nblk, 1, r->stream, r->timing ? r : nullptr};
+    T.side = r->side;
+    T.fork = r->ev_fork;
+    T.join = r->ev_join;
+    return T;
+}
+
+// Starts the injection clock of the coming cc on the side stream.
+void seq_clock_fork(const Target &T, int use_host_t, double t)
+{
+    (void)hipEventRecord(T.fork, T.stream);
+    (void)hipStreamWaitEvent(T.side, T.fork, 0);
+    launch_clock(T.Lside(kShapeRun), use_host_t, t);
+    (void)hipEventRecord(T.join, T.side);
 }
 
 struct TimeScope {
@@ -163,23 +189,32 @@ void seq_solve(const Target &T, int use_args, int mode, double tol, int min_swee
     launch_solver_begin(T.L(kShapeRun), use_args, mode, tol, min_sweeps, max_sweeps, check);
     Launch N = T.L(kShapeNodes);
     if (mode == 0) {
-        for (int s = 0; s < max_sweeps; s++) launch_jacobi_sweep(N, s, 0, -1, 0);
-        launch_solver_end(T.L(kShapeRun), 0);
+        for (int s = 0; s < max_sweeps; s++) launch_jacobi_sweep(N, s);
+        launch_solver_end(T.L(kShapeRun), 0, max_sweeps);
         return;
     }
-    int c = 0, s = 0;
+    Launch B = T.L(kShapeJacBlk);
     if (check < 1) check = 1;
+    int c = 0, s = 0, nlaunch = 0, dpar = 0;
     for (;;) {
-        if (mode == 2 && c > 0) launch_ir_update(N, s);
-        launch_residual(N, c, s, mode);
+        if (mode == 2 && c > 0) launch_ir_update(N, dpar);
+        launch_residual(N, c, mode, nlaunch & 1);
         if (s >= max_sweeps) break;
         int k = std::min(check, max_sweeps - s);
-        for (int q = 0; q < k; q++)
-            launch_jacobi_sweep(N, s + q, mode, q == 0 ? c : -1, (mode == 2 && q == 0) ? 1 : 0);
+        int dl = 0;
+        for (int q = 0; q < k;) {
+            int nsw = std::min(kJacH, k - q);
+            launch_jacobi_block(B, s + q, nsw, mode, q == 0 ? c : -1,
+                                (mode == 2 && q == 0) ? 1 : 0, mode == 2 ? (dl & 1) : (nlaunch & 1));
+            nlaunch++;
+            dl++;
+            q += nsw;
+        }
+        dpar = dl & 1;
         s += k;
         c += 1;
     }
-    launch_solver_end(T.L(kShapeRun), c);
+    launch_solver_end(T.L(kShapeRun), c, nlaunch & 1);
 }
 
 void seq_rho(const Target &T)
@@ -200,19 +235,23 @@ void seq_mover(const Target &T)
     launch_push_rank(T.L(kShapeTasks));
 }
 
-void seq_cc(const Target &T, int use_host_t, double t, int advance_t)
+void seq_cc(const Target &T, int use_host_t, double t, int advance_t, bool clock_forked)
 {
     {
         TimeScope ts(T, kTInject);
-        launch_clock(T.L(kShapeRun), use_host_t, t);
+        if (clock_forked)
+            (void)hipStreamWaitEvent(T.stream, T.join, 0);
+        else
+            launch_clock(T.L(kShapeRun), use_host_t, t);
         launch_inject(T.L(kShapeInject));
-        launch_far_rank(T.L(kShapeFar));
     }
     {
         TimeScope ts(T, kTScan);
         launch_scan_a(T.L(kShapeCellBlk));
         launch_scan_b(T.L(kShapeRunSp));
         launch_scan_c(T.L(kShapeCellBlk));
+        launch_far_place(T.L(kShapeFar));
+        launch_far_rank(T.L(kShapeFar));
     }
     {
         TimeScope ts(T, kTScatter);
@@ -461,10 +500,10 @@ int cupic_run_footprint(const cupic_params *pin, int64_t capacity_hint, int64_t 
     int64_t b = 0;
     b += 2 * 2 * 4 * z.cap * 4;                  // particles
     b += 2 * 2 * (z.ncells + 1) * 4;             // offsets
-    b += 2 * z.ncells * (9 + 10 + 1 + 1) * 4;    // cnt, base, far_cnt, tot
+    b += 2 * z.ncells * (9 + 10 + 1 + 1 + 1) * 4;  // cnt, base, far_cnt, far_cur, tot
     b += 2 * z.far_cap * (int64_t)sizeof(FarRec) + 2 * z.max_inject * 8;
     b += z.nnodes * 60;                          // grids
-    b += (int64_t)(p.jacobi_max_sweeps + 2) * 12;
+    b += (int64_t)(p.jacobi_max_sweeps + 2) * 12 * kRedSlots;
     *bytes = b;
     return CUPIC_OK;
 }
@@ -539,6 +578,7 @@ int cupic_run_create(const cupic_params *pin, int64_t capacity_hint, cupic_run *
         h.cnt[s] = dev_alloc<int>(r, z.ncells * 9, &st);
         h.base[s] = dev_alloc<int>(r, z.ncells * 10, &st);
         h.far_cnt[s] = dev_alloc<int>(r, z.ncells, &st);
+        h.far_cur[s] = dev_alloc<int>(r, z.ncells, &st);
         h.far[s] = dev_alloc<FarRec>(r, z.far_cap, &st);
         h.inj_tin[s] = dev_alloc<double>(r, z.max_inject, &st);
         h.tot[s] = dev_alloc<int>(r, z.ncells, &st);
@@ -555,8 +595,8 @@ int cupic_run_create(const cupic_params *pin, int64_t capacity_hint, cupic_run *
     h.d32[1] = dev_alloc<float>(r, z.nnodes, &st);
     h.r32 = dev_alloc<float>(r, z.nnodes, &st);
     h.E = dev_alloc<float2>(r, z.nnodes, &st);
-    h.sweep_max = dev_alloc<unsigned int>(r, (size_t)p.jacobi_max_sweeps + 1, &st);
-    h.resid = dev_alloc<unsigned long long>(r, (size_t)p.jacobi_max_sweeps + 2, &st);
+    h.sweep_max = dev_alloc<unsigned int>(r, ((size_t)p.jacobi_max_sweeps + 1) * kRedSlots, &st);
+    h.resid = dev_alloc<unsigned long long>(r, ((size_t)p.jacobi_max_sweeps + 2) * kRedSlots, &st);
     h.st = dev_alloc<RunState>(r, 1, &st);
     r->d_rd = dev_alloc<RunDev>(r, 1, &st);
     r->d_plan = dev_alloc<int>(r, (size_t)kShapeCount * 2, &st);
@@ -571,6 +611,9 @@ int cupic_run_create(const cupic_params *pin, int64_t capacity_hint, cupic_run *
         plan[2 * s + 1] = r->nblk[s];
     }
     if (hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&r->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&r->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&r->ev_join, hipEventDisableTiming) != hipSuccess ||
         hipMemcpy(r->d_rd, &h, sizeof(RunDev), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(r->d_plan, plan.data(), plan.size() * sizeof(int), hipMemcpyHostToDevice) !=
             hipSuccess ||
@@ -588,7 +631,11 @@ int cupic_run_destroy(cupic_run *r)
     if (r->stream) (void)hipStreamSynchronize(r->stream);
     for (auto &e : r->evs) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
     for (auto e : r->pool) (void)hipEventDestroy(e);
+    if (r->side) (void)hipStreamSynchronize(r->side);
     for (void *p : r->allocs) (void)hipFree(p);
+    if (r->ev_fork) (void)hipEventDestroy(r->ev_fork);
+    if (r->ev_join) (void)hipEventDestroy(r->ev_join);
+    if (r->side) (void)hipStreamDestroy(r->side);
     if (r->stream) (void)hipStreamDestroy(r->stream);
     delete r;
     return CUPIC_OK;
@@ -717,7 +764,8 @@ int cupic_cc(cupic_run *r, double t)
 {
     if (!r) return set_err(nullptr, CUPIC_ERR_ARG, "null run");
     if (!r->moved) return set_err(&r->err, CUPIC_ERR_STATE, "cc without particle_mover");
-    seq_cc(target_of(r), 1, t, 0);
+    seq_cc(target_of(r), 1, t, 0, r->clock_forked);
+    r->clock_forked = false;
     r->moved = 0;
     r->acc_valid = true;
     r->t_next = t + r->p.dt;
@@ -726,7 +774,13 @@ int cupic_cc(cupic_run *r, double t)
 
 int cupic_step(cupic_run *r, double t)
 {
+    if (!r) return set_err(nullptr, CUPIC_ERR_ARG, "null run");
+    if (r->moved) return set_err(&r->err, CUPIC_ERR_STATE, "step between mover and cc");
     int st = cupic_charge_deposition(r);
+    if (!st) {
+        seq_clock_fork(target_of(r), 1, t);
+        r->clock_forked = true;
+    }
     if (!st) {
         seq_solve(target_of(r), 0, r->p.jacobi_mode, 0, 0, r->p.jacobi_max_sweeps,
                   r->p.jacobi_check_every);
@@ -1045,6 +1099,34 @@ const char *cupic_timing_names(void) { return kTimeNames; }
 
 // ---- batches
 
+}  // extern "C"
+
+namespace {
+
+Target batch_target(cupic_batch *b)
+{
+    Target T{b->d_rds, b->d_plan, b->nblk, (int)b->runs.size(), b->stream, nullptr};
+    T.side = b->side;
+    T.fork = b->ev_fork;
+    T.join = b->ev_join;
+    return T;
+}
+
+// One step of every run in the batch; t lives on the device and advances.
+void batch_step_sequence(cupic_batch *b, const Target &T)
+{
+    launch_rho_convert(T.L(kShapeNodes));
+    seq_clock_fork(T, 0, 0.0);
+    seq_solve(T, 0, b->mode, 0, 0, b->max_sweeps, b->check);
+    launch_field(T.L(kShapeNodes));
+    launch_push_rank(T.L(kShapeTasks));
+    seq_cc(T, 0, 0.0, 1, true);
+}
+
+}  // namespace
+
+extern "C" {
+
 int cupic_batch_create(cupic_run *const *runs, int32_t nruns, int32_t use_graph,
                        cupic_batch **out)
 {
@@ -1103,15 +1185,14 @@ int cupic_batch_create(cupic_run *const *runs, int32_t nruns, int32_t use_graph,
     HIPTRY(nullptr, hipMemcpy(b->d_rds, hs.data(), hs.size() * sizeof(RunDev), hipMemcpyHostToDevice));
     HIPTRY(nullptr, hipMemcpy(b->d_plan, plan.data(), plan.size() * sizeof(int), hipMemcpyHostToDevice));
     HIPTRY(nullptr, hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    HIPTRY(nullptr, hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking));
+    HIPTRY(nullptr, hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming));
+    HIPTRY(nullptr, hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming));
     HIPTRY(nullptr, hipDeviceSynchronize());
     if (b->use_graph) {
-        Target T{b->d_rds, b->d_plan, b->nblk, nruns, b->stream, nullptr};
+        Target T = batch_target(b);
         HIPTRY(&b->err, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-        launch_rho_convert(T.L(kShapeNodes));
-        seq_solve(T, 0, b->mode, 0, 0, b->max_sweeps, b->check);
-        launch_field(T.L(kShapeNodes));
-        launch_push_rank(T.L(kShapeTasks));
-        seq_cc(T, 0, 0.0, 1);
+        batch_step_sequence(b, T);
         HIPTRY(&b->err, hipStreamEndCapture(b->stream, &b->graph));
         HIPTRY(&b->err, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
     }
@@ -1122,17 +1203,12 @@ int cupic_batch_create(cupic_run *const *runs, int32_t nruns, int32_t use_graph,
 int cupic_batch_step(cupic_batch *b, int32_t nsteps)
 {
     if (!b) return set_err(nullptr, CUPIC_ERR_ARG, "null batch");
-    int nruns = (int)b->runs.size();
-    Target T{b->d_rds, b->d_plan, b->nblk, nruns, b->stream, nullptr};
+    Target T = batch_target(b);
     for (int k = 0; k < nsteps; k++) {
         if (b->use_graph) {
             HIPTRY(&b->err, hipGraphLaunch(b->exec, b->stream));
         } else {
-            launch_rho_convert(T.L(kShapeNodes));
-            seq_solve(T, 0, b->mode, 0, 0, b->max_sweeps, b->check);
-            launch_field(T.L(kShapeNodes));
-            launch_push_rank(T.L(kShapeTasks));
-            seq_cc(T, 0, 0.0, 1);
+            batch_step_sequence(b, T);
         }
     }
     hipError_t e = hipGetLastError();
@@ -1152,6 +1228,10 @@ int cupic_batch_destroy(cupic_batch *b)
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
+    if (b->side) (void)hipStreamSynchronize(b->side);
+    if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
+    if (b->ev_join) (void)hipEventDestroy(b->ev_join);
+    if (b->side) (void)hipStreamDestroy(b->side);
     if (b->d_rds) (void)hipFree(b->d_rds);
     if (b->d_plan) (void)hipFree(b->d_plan);
     if (b->stream) (void)hipStreamDestroy(b->stream);

@@ -27,6 +27,10 @@ constexpr int kBlock = 256;
 constexpr int kCmax = 16;              // max cells per wave task
 constexpr int kTileW = kCmax + 3;      // deposit tile width (node columns)
 constexpr int kScanCells = 1024;       // cells per scan block
+constexpr int kRedSlots = 16;         // atomic slots per reduced quantity
+constexpr int kJacTX = 64;             // blocked Jacobi tile (x)
+constexpr int kJacTY = 16;             // blocked Jacobi tile (y)
+constexpr int kJacH = 10;              // sweeps per blocked launch = LDS halo
 
 enum Dir : int { kDirFar = 9, kDirProbe = 10, kDirTop = 11, kDirNone = 15 };
 
@@ -51,8 +55,7 @@ struct RunState {
     int N[2];                       // particles per species (current buffer)
     int N_new[2];
     int n_in[2];                    // injected this step
-    int inj_base[2];                // first far-list slot of injected particles
-    int far_n[2];                   // far-list length this step
+    int far_n[2];                   // fast movers in the far list (injected follow)
     int tin_started;
     int step;
     double t;
@@ -63,6 +66,7 @@ struct RunState {
     int solve_mode, solve_min, solve_max, solve_check;
     double solve_tol;
     int solve_done;
+    int phi_final;                  // phi buffer holding the solution
     int sweeps_done;
     double last_resid;
     long long sweeps_total;         // sweeps over all solves
@@ -102,9 +106,10 @@ struct RunDev {
     unsigned int quirks;
     float *px[2][2], *py[2][2], *pvx[2][2], *pvy[2][2];   // [species][buffer]
     int *off[2][2];                 // [species][buffer], ncells + 1
-    int *cnt[2];                    // ncells * 9
-    int *base[2];                   // ncells * 10
-    int *far_cnt[2];                // ncells
+    int *cnt[2];                    // [9][ncells] arrivals per (direction, old cell)
+    int *base[2];                   // [10][ncells] slot bases per new cell
+    int *far_cnt[2];                // ncells, far entries per new cell
+    int *far_cur[2];                // ncells, placement cursors
     FarRec *far[2];                 // far_cap
     double *inj_tin[2];             // max_inject
     int *tot[2];                    // ncells
@@ -118,8 +123,8 @@ struct RunDev {
     float *d32[2];                  // nnodes, mode 2 correction
     float *r32;                     // nnodes, mode 2 correction rhs
     float2 *E;                      // nnodes
-    unsigned int *sweep_max;        // max_sweeps_alloc + 1 (float bits)
-    unsigned long long *resid;      // max_sweeps_alloc + 2 (double bits)
+    unsigned int *sweep_max;        // (max_sweeps_alloc + 1) * kRedSlots (float bits)
+    unsigned long long *resid;      // (max_sweeps_alloc + 2) * kRedSlots (double bits)
     RunState *st;
 };
 

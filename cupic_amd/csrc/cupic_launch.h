@@ -30,6 +30,7 @@ enum Shape : int {
     kShapeInject,      // 2 * ceil(max_inject / 256)
     kShapeRows,        // 2 * ncy
     kShapeInit,        // 2 * ceil(n_init / 256)
+    kShapeJacBlk,      // ceil(ncx / kJacTX) * ceil((ncy - 1) / kJacTY)
     kShapeCount
 };
 
@@ -40,10 +41,12 @@ void launch_phi_linear(const Launch &L);
 void launch_rho_convert(const Launch &L);
 void launch_solver_begin(const Launch &L, int use_args, int mode, double tol, int min_sweeps,
                          int max_sweeps, int check_every);
-void launch_jacobi_sweep(const Launch &L, int s, int mode, int check_slot, int first);
-void launch_ir_update(const Launch &L, int s);
-void launch_residual(const Launch &L, int c, int s, int mode);
-void launch_solver_end(const Launch &L, int last_check);
+void launch_jacobi_sweep(const Launch &L, int s);
+void launch_jacobi_block(const Launch &L, int s0, int nsw, int mode, int check_slot, int first,
+                         int par);
+void launch_ir_update(const Launch &L, int par);
+void launch_residual(const Launch &L, int c, int mode, int par);
+void launch_solver_end(const Launch &L, int last_check, int par);
 void launch_field(const Launch &L);
 
 // particles
@@ -57,6 +60,7 @@ void launch_scatter(const Launch &L);
 // boundary conditions and sort
 void launch_clock(const Launch &L, int use_host_t, double t_host);
 void launch_inject(const Launch &L);
+void launch_far_place(const Launch &L);
 void launch_far_rank(const Launch &L);
 void launch_scan_a(const Launch &L);
 void launch_scan_b(const Launch &L);

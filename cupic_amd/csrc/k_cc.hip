@@ -22,7 +22,8 @@ constexpr double kPi = 3.1415926535897932;  // src/stdh.h:26
 // cc call as t + dtin; this step inserts in = 1 + int((t+dt-tin)/dtin)
 // particles when tin < t+dt, advancing tin by dtin once per particle.  The
 // recurrence runs serially in FP64 so the counts match the reference's
-// host loop bit for bit.
+// host loop bit for bit.  It reads only the clock, so a step can run it on a
+// side stream while the field is solved.
 __global__ __launch_bounds__(kBlock) void k_clock(const RunDev *rds, const int *start,
                                                   int nruns, int use_host_t, double t_host)
 {
@@ -54,12 +55,8 @@ __global__ __launch_bounds__(kBlock) void k_clock(const RunDev *rds, const int *
     unsigned int err = 0;
     if (ne > R.max_inject) { ne = R.max_inject; err |= kErrInjectCap; }
     if (st->N[s] + ne > R.cap) { ne = max(0, R.cap - st->N[s]); err |= kErrCapacity; }
-    int fb = st->far_n[s];
-    if (fb + ne > R.far_cap) { ne = max(0, R.far_cap - fb); err |= kErrFarOverflow; }
     if (err) atomicOr(&st->err, err);
     st->n_in[s] = ne;
-    st->inj_base[s] = fb;
-    st->far_n[s] = fb + ne;
     st->ninj_first[s] = st->ninj[s];
     st->ninj[s] += (unsigned long long)ne;
 }
@@ -130,48 +127,18 @@ __global__ __launch_bounds__(kBlock) void k_inject(const RunDev *rds, const int 
     rec.idx = st->N[s] + k;
     rec.rank = 0;
     rec.pad = 0;
-    R.far[s][st->inj_base[s] + k] = rec;
+    int slot = st->far_n[s] + k;  // after the fast movers of pass 1
+    if (slot >= R.far_cap) {
+        atomicOr(&st->err, kErrFarOverflow);
+        return;
+    }
+    R.far[s][slot] = rec;
+    atomicAdd(&R.far_cnt[s][rec.key], 1);
 }
 
-// Stable rank of each far-list entry within its new cell: the number of
-// entries with the same cell and a smaller input index.  The list is short
-// (injected particles plus rare fast movers), so entries are compared
-// directly, 256 at a time through LDS; far_cnt counts entries per cell.
-__global__ __launch_bounds__(kBlock) void k_far_rank(const RunDev *rds, const int *start,
-                                                     int nruns)
-{
-    RUN_PROLOGUE();
-    __shared__ int skey[kBlock];
-    __shared__ int sidx[kBlock];
-    int bps = (R.far_cap + kBlock - 1) / kBlock;
-    int s = local / bps;
-    if (s > 1) return;
-    int e0 = (local - s * bps) * kBlock;
-    int F = min(st->far_n[s], R.far_cap);
-    if (e0 >= F) return;  // block-uniform
-    int e = e0 + threadIdx.x;
-    FarRec *far = R.far[s];
-    int key = -1, idx = 0;
-    if (e < F) { key = far[e].key; idx = far[e].idx; }
-    int rank = 0;
-    for (int t0 = 0; t0 < F; t0 += kBlock) {
-        int q = t0 + threadIdx.x;
-        __syncthreads();
-        skey[threadIdx.x] = q < F ? far[q].key : -2;
-        sidx[threadIdx.x] = q < F ? far[q].idx : 0;
-        __syncthreads();
-        int m = min(kBlock, F - t0);
-        for (int u = 0; u < m; u++) rank += (skey[u] == key && sidx[u] < idx);
-    }
-    if (e < F) {
-        far[e].rank = rank;
-        atomicAdd(&R.far_cnt[s][key], 1);
-    }
-}
-
-// Count of particles arriving in new cell k from neighbour slot sl (source
-// offset (sl%3-1, sl/3-1), periodic in x): the source cell's count for the
-// direction that leads to k.
+// Count of particles arriving in new cell (i, j) from neighbour slot sl
+// (source offset (sl%3-1, sl/3-1), periodic in x): the source cell's count
+// for the direction that leads here.  cnt is direction-major.
 __device__ __forceinline__ int src_count(const RunDev &R, const int *cnt, int i, int j, int sl)
 {
     int dj = sl / 3 - 1, di = sl - 3 * (sl / 3) - 1;
@@ -180,38 +147,47 @@ __device__ __forceinline__ int src_count(const RunDev &R, const int *cnt, int i,
     int ii = i + di;
     if (ii < 0) ii += R.ncx;
     if (ii >= R.ncx) ii -= R.ncx;
-    return cnt[(jj * R.ncx + ii) * 9 + (8 - sl)];
+    return cnt[(8 - sl) * R.ncells + jj * R.ncx + ii];
 }
 
+// Exclusive scan of one value per thread over the block; returns the
+// exclusive prefix, total in *total.  Wave scans, then the wave totals.
 __device__ __forceinline__ int block_excl_scan(int v, int *sh, int &total)
 {
-    // Hillis-Steele over 256 entries in LDS
-    sh[threadIdx.x] = v;
-    __syncthreads();
-    for (int o = 1; o < kBlock; o <<= 1) {
-        int a = threadIdx.x >= (unsigned)o ? sh[threadIdx.x - o] : 0;
-        __syncthreads();
-        sh[threadIdx.x] += a;
-        __syncthreads();
+    int lane = threadIdx.x & (kWave - 1), w = threadIdx.x >> 6;
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        int u = __shfl_up(x, o, kWave);
+        if (lane >= o) x += u;
     }
-    total = sh[kBlock - 1];
-    int incl = sh[threadIdx.x];
+    if (lane == kWave - 1) sh[w] = x;
     __syncthreads();
-    return incl - v;
+    int before = 0, tot = 0;
+#pragma unroll
+    for (int q = 0; q < kBlock / kWave; q++) {
+        int t = sh[q];
+        if (q < w) before += t;
+        tot += t;
+    }
+    __syncthreads();
+    total = tot;
+    return before + x - v;
 }
 
-// Per-cell totals of the new state and per-block sums.
+// Per-cell totals of the new state (near arrivals plus the far list) and
+// per-block sums.  One thread per cell, 1024 cells per block.
 __global__ __launch_bounds__(kBlock) void k_scan_a(const RunDev *rds, const int *start,
                                                    int nruns)
 {
     RUN_PROLOGUE();
-    __shared__ int sh[kBlock];
+    __shared__ int sh[kBlock / kWave];
     int nblk = (R.ncells + kScanCells - 1) / kScanCells;
     int s = local / nblk;
     int blk = local - s * nblk;
     int sum = 0;
     for (int u = 0; u < kScanCells / kBlock; u++) {
-        int k = blk * kScanCells + threadIdx.x * (kScanCells / kBlock) + u;
+        int k = blk * kScanCells + u * kBlock + threadIdx.x;
         if (k >= R.ncells) break;
         int j = k / R.ncx, i = k - j * R.ncx;
         int t = R.far_cnt[s][k];
@@ -230,7 +206,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_b(const RunDev *rds, const int 
                                                    int nruns)
 {
     RUN_PROLOGUE();
-    __shared__ int sh[kBlock];
+    __shared__ int sh[kBlock / kWave];
     int s = local;
     if (s > 1) return;
     int nblk = (R.ncells + kScanCells - 1) / kScanCells;
@@ -246,43 +222,86 @@ __global__ __launch_bounds__(kBlock) void k_scan_b(const RunDev *rds, const int 
     if (threadIdx.x == 0) st->N_new[s] = carry;
 }
 
-// New offsets and slot bases: base[k*10+sl] = off[k] + arrivals from slots
-// before sl; slot 9 (far list) last.  Clears far_cnt for the next step.
+// New offsets and slot bases (slot-major): base[sl][k] = off[k] + arrivals
+// from slots before sl; slot 9 (far list) last.  Clears the far counters.
 __global__ __launch_bounds__(kBlock) void k_scan_c(const RunDev *rds, const int *start,
                                                    int nruns)
 {
     RUN_PROLOGUE();
-    __shared__ int sh[kBlock];
+    __shared__ int sh[kBlock / kWave];
     int nblk = (R.ncells + kScanCells - 1) / kScanCells;
     int s = local / nblk;
     int blk = local - s * nblk;
-    constexpr int per = kScanCells / kBlock;
-    int k0 = blk * kScanCells + threadIdx.x * per;
-    int sum = 0;
-    for (int u = 0; u < per; u++) {
-        int k = k0 + u;
-        if (k < R.ncells) sum += R.tot[s][k];
-    }
-    int total;
-    int run = block_excl_scan(sum, sh, total) + R.blk[s][blk];
     int *noff = R.off[s][st->cur ^ 1];
     int *base = R.base[s];
-    for (int u = 0; u < per; u++) {
-        int k = k0 + u;
-        if (k >= R.ncells) break;
-        int j = k / R.ncx, i = k - j * R.ncx;
-        noff[k] = run;
-        int b = run;
+    int carry = R.blk[s][blk];
+    for (int u = 0; u < kScanCells / kBlock; u++) {
+        int k = blk * kScanCells + u * kBlock + threadIdx.x;
+        bool ok = k < R.ncells;
+        int t = ok ? R.tot[s][k] : 0;
+        int total;
+        int ex = block_excl_scan(t, sh, total);
+        if (ok) {
+            int j = k / R.ncx, i = k - j * R.ncx;
+            int b = carry + ex;
+            noff[k] = b;
 #pragma unroll
-        for (int sl = 0; sl < 9; sl++) {
-            base[k * 10 + sl] = b;
-            b += src_count(R, R.cnt[s], i, j, sl);
+            for (int sl = 0; sl < 9; sl++) {
+                base[sl * R.ncells + k] = b;
+                b += src_count(R, R.cnt[s], i, j, sl);
+            }
+            base[9 * R.ncells + k] = b;
+            R.far_cnt[s][k] = 0;
+            R.far_cur[s][k] = 0;
         }
-        base[k * 10 + 9] = b;
-        R.far_cnt[s][k] = 0;
-        run += R.tot[s][k];
+        carry += total;
     }
-    if (blk == nblk - 1 && threadIdx.x == kBlock - 1) noff[R.ncells] = st->N_new[s];
+    if (blk == nblk - 1 && threadIdx.x == 0) noff[R.ncells] = st->N_new[s];
+}
+
+// Far list, after the scan: each entry takes a place in its cell's far
+// region (any order) and writes its list index there, in the output buffer
+// that the far scatter overwrites last.
+__global__ __launch_bounds__(kBlock) void k_far_place(const RunDev *rds, const int *start,
+                                                      int nruns)
+{
+    RUN_PROLOGUE();
+    int bps = (R.far_cap + kBlock - 1) / kBlock;
+    int s = local / bps;
+    if (s > 1) return;
+    int e = (local - s * bps) * kBlock + threadIdx.x;
+    int F = min(st->far_n[s] + st->n_in[s], R.far_cap);
+    if (e >= F) return;
+    int key = R.far[s][e].key;
+    int slot = atomicAdd(&R.far_cur[s][key], 1);
+    int pos = R.base[s][9 * R.ncells + key] + slot;
+    if ((unsigned)pos < (unsigned)R.cap)
+        ((int *)R.py[s][st->cur ^ 1])[pos] = e;
+}
+
+// Stable rank of a far entry within its cell: entries of the same cell with
+// a smaller input index (a handful per cell).
+__global__ __launch_bounds__(kBlock) void k_far_rank(const RunDev *rds, const int *start,
+                                                     int nruns)
+{
+    RUN_PROLOGUE();
+    int bps = (R.far_cap + kBlock - 1) / kBlock;
+    int s = local / bps;
+    if (s > 1) return;
+    int e = (local - s * bps) * kBlock + threadIdx.x;
+    int F = min(st->far_n[s] + st->n_in[s], R.far_cap);
+    if (e >= F) return;
+    const FarRec *far = R.far[s];
+    int key = far[e].key, idx = far[e].idx;
+    int b = R.base[s][9 * R.ncells + key];
+    int n = R.off[s][st->cur ^ 1][key + 1] - b;
+    const int *slots = (const int *)R.py[s][st->cur ^ 1];
+    int rank = 0;
+    for (int u = 0; u < n; u++) {
+        int q = slots[b + u];
+        if ((unsigned)q < (unsigned)F) rank += far[q].idx < idx;
+    }
+    R.far[s][e].rank = rank;
 }
 
 // Far-list entries to their sorted positions, with their charge (global
@@ -295,11 +314,11 @@ __global__ __launch_bounds__(kBlock) void k_far_scatter(const RunDev *rds, const
     int s = local / bps;
     if (s > 1) return;
     int e = (local - s * bps) * kBlock + threadIdx.x;
-    int F = min(st->far_n[s], R.far_cap);
+    int F = min(st->far_n[s] + st->n_in[s], R.far_cap);
     if (e >= F) return;
     FarRec rec = R.far[s][e];
     int o = st->cur ^ 1;
-    int dest = R.base[s][rec.key * 10 + 9] + rec.rank;
+    int dest = R.base[s][9 * R.ncells + rec.key] + rec.rank;
     if ((unsigned)dest >= (unsigned)R.cap) {
         atomicOr(&st->err, kErrSortInvariant);
         return;
@@ -392,6 +411,7 @@ void launch_clock(const Launch &L, int use_host_t, double t_host)
     LAUNCH_K(k_clock, L, use_host_t, t_host);
 }
 void launch_inject(const Launch &L) { LAUNCH_K(k_inject, L); }
+void launch_far_place(const Launch &L) { LAUNCH_K(k_far_place, L); }
 void launch_far_rank(const Launch &L) { LAUNCH_K(k_far_rank, L); }
 void launch_scan_a(const Launch &L) { LAUNCH_K(k_scan_a, L); }
 void launch_scan_b(const Launch &L) { LAUNCH_K(k_scan_b, L); }

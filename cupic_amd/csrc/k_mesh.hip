@@ -59,8 +59,10 @@ __global__ __launch_bounds__(kBlock) void k_solver_begin(const RunDev *rds, cons
     RUN_PROLOGUE();
     int mx = use_args ? max_sweeps : R.step_max;
     if (mx > R.max_sweeps_alloc) mx = R.max_sweeps_alloc;
-    for (int q = threadIdx.x; q <= R.max_sweeps_alloc; q += kBlock) R.sweep_max[q] = 0u;
-    for (int q = threadIdx.x; q <= R.max_sweeps_alloc + 1; q += kBlock) R.resid[q] = 0ull;
+    for (int q = threadIdx.x; q < (R.max_sweeps_alloc + 1) * kRedSlots; q += kBlock)
+        R.sweep_max[q] = 0u;
+    for (int q = threadIdx.x; q < (R.max_sweeps_alloc + 2) * kRedSlots; q += kBlock)
+        R.resid[q] = 0ull;
     if (threadIdx.x == 0) {
         st->solve_mode = use_args ? mode : R.step_mode;
         st->solve_tol = use_args ? tol : R.step_tol;
@@ -70,49 +72,78 @@ __global__ __launch_bounds__(kBlock) void k_solver_begin(const RunDev *rds, cons
         st->solve_done = 0;
         st->sweeps_done = 0;
         st->last_resid = 0.0;
+        st->phi_final = st->phi_cur;
     }
 }
 
+// Reductions go to kRedSlots addresses per quantity (block b uses slot
+// b % kRedSlots) so that thousands of blocks do not serialise on one
+// atomic; readers take the max over the slots.
+__device__ __forceinline__ double read_resid(const RunDev &R, int c)
+{
+    double m = 0.0;
+#pragma unroll
+    for (int q = 0; q < kRedSlots; q++)
+        m = fmax(m, __longlong_as_double((long long)R.resid[c * kRedSlots + q]));
+    return m;
+}
+
+__device__ __forceinline__ float read_sweep_max(const RunDev &R, int s)
+{
+    float m = 0.0f;
+#pragma unroll
+    for (int q = 0; q < kRedSlots; q++) m = fmaxf(m, __uint_as_float(R.sweep_max[s * kRedSlots + q]));
+    return m;
+}
+
 __device__ __forceinline__ void mark_done(const RunDev &R, RunState *st, int local, int s,
-                                          double res)
+                                          double res, int par)
 {
     if (local == 0 && threadIdx.x == 0) {
         st->sweeps_done = s;
         st->last_resid = res;
+        st->phi_final = (st->phi_cur + par) & 1;
         st->solve_done = 1;
     }
 }
 
+// block-wide max of one value per thread, into a slotted atomic
+template <class T, class F>
+__device__ __forceinline__ void block_max_atomic(T v, T *sh, F emit)
+{
+    int lane = threadIdx.x & (kWave - 1), w = threadIdx.x >> 6;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        T u = __shfl_xor(v, o, kWave);
+        v = v > u ? v : u;
+    }
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        T m = sh[0];
+        for (int q = 1; q < kBlock / kWave; q++) m = m > sh[q] ? m : sh[q];
+        emit(m);
+    }
+}
+
 // One Jacobi sweep s on rows 1..ncy-1 (src/mesh.cu:297-381), double
-// buffered (Q3), periodic in x.  Modes 0/1 relax phi itself with rhs b;
-// mode 2 relaxes the correction d with rhs r (d starts at 0 each cycle).
-// Convergence is decided on the device: mode 0 reads the previous sweep's
-// max |dphi|, modes 1/2 the FP64 residual of the check that precedes this
-// block of sweeps.
+// buffered (Q3), periodic in x, with the reference stopping rule (mode 0):
+// the sweep first reads the previous sweep's max |dphi| and stops the solve
+// once it is below tol after at least solve_min sweeps.
 __global__ __launch_bounds__(kBlock) void k_jacobi_sweep(const RunDev *rds, const int *start,
-                                                         int nruns, int s, int mode,
-                                                         int check_slot, int first)
+                                                         int nruns, int s)
 {
     RUN_PROLOGUE();
+    __shared__ float sh[kBlock / kWave];
     if (st->solve_done) return;
-    if (mode == 0) {
-        if (s > 0 && s >= st->solve_min &&
-            __uint_as_float(R.sweep_max[s - 1]) < (float)st->solve_tol) {
-            mark_done(R, st, local, s, (double)__uint_as_float(R.sweep_max[s - 1]));
-            return;
-        }
-        if (s >= st->solve_max) {
-            mark_done(R, st, local, s, s > 0 ? (double)__uint_as_float(R.sweep_max[s - 1]) : 0.0);
-            return;
-        }
-    } else if (check_slot >= 0) {
-        double res = __longlong_as_double((long long)R.resid[check_slot]);
-        if (res <= st->solve_tol || s >= st->solve_max) {
-            mark_done(R, st, local, s, res);
-            return;
-        }
+    if (s > 0 && s >= st->solve_min && (double)read_sweep_max(R, s - 1) < st->solve_tol) {
+        mark_done(R, st, local, s, (double)read_sweep_max(R, s - 1), s);
+        return;
     }
-    if (s >= st->solve_max) return;
+    if (s >= st->solve_max) {
+        mark_done(R, st, local, s, s > 0 ? (double)read_sweep_max(R, s - 1) : 0.0, s);
+        return;
+    }
     int k = local * kBlock + threadIdx.x;
     float dmax = 0.0f;
     if (k < R.nnodes) {
@@ -122,38 +153,101 @@ __global__ __launch_bounds__(kBlock) void k_jacobi_sweep(const RunDev *rds, cons
         if (j >= 1 && j < R.ncy) {
             int kw = (i == 0) ? k + ncx - 1 : k - 1;
             int ke = (i == ncx - 1) ? k - ncx + 1 : k + 1;
-            if (mode == 2) {
-                const float *din = R.d32[s & 1];
-                float *dout = R.d32[(s + 1) & 1];
-                float rr = R.r32[k];
-                float v;
-                if (first) {
-                    v = 0.25f * rr;
-                } else {
-                    v = 0.25f * ((((rr + din[kw]) + din[ke]) + din[k + ncx]) + din[k - ncx]);
-                }
-                dout[k] = v;
-            } else {
-                int pc = st->phi_cur;
-                const float *in = R.phi32[(pc + s) & 1];
-                float *out = R.phi32[(pc + s + 1) & 1];
-                float c = in[k];
-                float v = 0.25f * ((((R.b32[k] + in[kw]) + in[ke]) + in[k + ncx]) + in[k - ncx]);
-                out[k] = v;
-                dmax = fabsf(v - c);
-            }
+            int pc = st->phi_cur;
+            const float *in = R.phi32[(pc + s) & 1];
+            float *out = R.phi32[(pc + s + 1) & 1];
+            float c = in[k];
+            float v = 0.25f * ((((R.b32[k] + in[kw]) + in[ke]) + in[k + ncx]) + in[k - ncx]);
+            out[k] = v;
+            dmax = fabsf(v - c);
         }
     }
-    if (mode == 0) {
-        dmax = wave_max_f(dmax);
-        if ((threadIdx.x & (kWave - 1)) == 0 && dmax > 0.0f)
-            atomicMax(&R.sweep_max[s], __float_as_uint(dmax));
+    unsigned int *slot = &R.sweep_max[s * kRedSlots + (local & (kRedSlots - 1))];
+    block_max_atomic(dmax, sh, [&](float m) {
+        if (m > 0.0f) atomicMax(slot, __float_as_uint(m));
+    });
+}
+
+// nsw <= kJacH Jacobi sweeps s0 .. s0+nsw-1 in one launch (modes 1, 2).
+// Each workgroup loads its kJacTX x kJacTY tile with a kJacH-node halo into
+// LDS, sweeps there (the valid region shrinks by one node per sweep and
+// still covers the tile after nsw sweeps) and writes the tile back: the
+// iterates equal those of nsw single sweeps.  Mode 1 relaxes phi (rhs b),
+// mode 2 the refinement correction d (rhs r), d starting at 0 when first.
+// par selects the input buffer; the output goes to the other one.
+__global__ __launch_bounds__(kBlock) void k_jacobi_block(const RunDev *rds, const int *start,
+                                                         int nruns, int s0, int nsw, int mode,
+                                                         int check_slot, int first, int par)
+{
+    RUN_PROLOGUE();
+    constexpr int LW = kJacTX + 2 * kJacH, LH = kJacTY + 2 * kJacH, LN = LW * LH;
+    __shared__ float A[2][LN];
+    __shared__ float Bv[LN];
+    if (st->solve_done) return;
+    if (check_slot >= 0) {
+        double res = read_resid(R, check_slot);
+        if (res <= st->solve_tol || s0 >= st->solve_max) {
+            mark_done(R, st, local, s0, res, par);
+            return;
+        }
+    }
+    if (s0 >= st->solve_max) return;
+    const int ncx = R.ncx, ncy = R.ncy;
+    const int tiles_x = (ncx + kJacTX - 1) / kJacTX;
+    const int by = local / tiles_x, bx = local - by * tiles_x;
+    const int gx0 = bx * kJacTX - kJacH, gy0 = 1 + by * kJacTY - kJacH;
+    const float *in, *rhs;
+    float *out;
+    if (mode == 2) {
+        in = R.d32[par & 1];
+        out = R.d32[(par + 1) & 1];
+        rhs = R.r32;
+    } else {
+        in = R.phi32[(st->phi_cur + par) & 1];
+        out = R.phi32[(st->phi_cur + par + 1) & 1];
+        rhs = R.b32;
+    }
+    for (int q = threadIdx.x; q < LN; q += kBlock) {
+        int lr = q / LW, lc = q - lr * LW;
+        int gy = gy0 + lr;
+        int gx = (gx0 + lc) % ncx;
+        if (gx < 0) gx += ncx;
+        float a = 0.0f, bb = 0.0f;
+        if (gy >= 0 && gy <= ncy) {
+            int g = gy * ncx + gx;
+            a = (mode == 2 && first) ? 0.0f : in[g];
+            bb = rhs[g];
+        }
+        A[0][q] = a;
+        A[1][q] = a;
+        Bv[q] = bb;
+    }
+    __syncthreads();
+    int cur = 0;
+    for (int j = 0; j < nsw; j++) {
+        const float *a = A[cur];
+        float *o = A[cur ^ 1];
+        for (int q = threadIdx.x; q < LN; q += kBlock) {
+            int lr = q / LW, lc = q - lr * LW;
+            int gy = gy0 + lr;
+            if (lr >= 1 && lr < LH - 1 && lc >= 1 && lc < LW - 1 && gy >= 1 && gy < ncy)
+                o[q] = 0.25f * ((((Bv[q] + a[q - 1]) + a[q + 1]) + a[q + LW]) + a[q - LW]);
+        }
+        __syncthreads();
+        cur ^= 1;
+    }
+    for (int q = threadIdx.x; q < kJacTX * kJacTY; q += kBlock) {
+        int tr = q / kJacTX, tc = q - tr * kJacTX;
+        int gy = 1 + by * kJacTY + tr, gx = bx * kJacTX + tc;
+        if (gy < ncy && gx < ncx)
+            out[gy * ncx + gx] = A[cur][(tr + kJacH) * LW + (tc + kJacH)];
     }
 }
 
-// mode 2: phi64 += d, the correction solved by the sweeps since check s.
+// mode 2: phi64 += d, the correction solved by the last cycle of sweeps
+// (d in buffer par).
 __global__ __launch_bounds__(kBlock) void k_ir_update(const RunDev *rds, const int *start,
-                                                      int nruns, int s)
+                                                      int nruns, int par)
 {
     RUN_PROLOGUE();
     if (st->solve_done) return;
@@ -161,17 +255,18 @@ __global__ __launch_bounds__(kBlock) void k_ir_update(const RunDev *rds, const i
     if (k >= R.nnodes) return;
     int j = k / R.ncx;
     if (j < 1 || j >= R.ncy) return;
-    R.phi64[k] += (double)R.d32[s & 1][k];
+    R.phi64[k] += (double)R.d32[par & 1][k];
 }
 
 // FP64 residual max over interior nodes of |0.25 (b + sum of neighbours) - phi|,
 // the same measure as the reference's max |dphi| (src/mesh.cu:354).  Mode 1
-// evaluates it on the FP32 phi; mode 2 on the FP64 master and also stores
-// the correction right-hand side r = b + sum - 4 phi.
+// evaluates it on the FP32 phi in buffer par; mode 2 on the FP64 master and
+// also stores the correction right-hand side r = b + sum - 4 phi.
 __global__ __launch_bounds__(kBlock) void k_residual(const RunDev *rds, const int *start,
-                                                     int nruns, int c, int s, int mode)
+                                                     int nruns, int c, int mode, int par)
 {
     RUN_PROLOGUE();
+    __shared__ double sh[kBlock / kWave];
     if (st->solve_done) return;
     int k = local * kBlock + threadIdx.x;
     double rmax = 0.0;
@@ -187,7 +282,7 @@ __global__ __launch_bounds__(kBlock) void k_residual(const RunDev *rds, const in
                 const double *p = R.phi64;
                 pc_ = p[k]; pw = p[kw]; pe = p[ke]; pn = p[k + ncx]; ps = p[k - ncx];
             } else {
-                const float *p = R.phi32[(st->phi_cur + s) & 1];
+                const float *p = R.phi32[(st->phi_cur + par) & 1];
                 pc_ = p[k]; pw = p[kw]; pe = p[ke]; pn = p[k + ncx]; ps = p[k - ncx];
             }
             double sum = (((R.b64[k] + pw) + pe) + pn) + ps;
@@ -196,14 +291,14 @@ __global__ __launch_bounds__(kBlock) void k_residual(const RunDev *rds, const in
             rmax = fabs(0.25 * rr);
         }
     }
-    rmax = wave_max_d(rmax);
-    if ((threadIdx.x & (kWave - 1)) == 0 && rmax > 0.0)
-        atomicMax(&R.resid[c], (unsigned long long)__double_as_longlong(rmax));
-    (void)s;
+    unsigned long long *slot = &R.resid[c * kRedSlots + (local & (kRedSlots - 1))];
+    block_max_atomic(rmax, sh, [&](double m) {
+        if (m > 0.0) atomicMax(slot, (unsigned long long)__double_as_longlong(m));
+    });
 }
 
 __global__ __launch_bounds__(kBlock) void k_solver_end(const RunDev *rds, const int *start,
-                                                       int nruns, int last_check)
+                                                       int nruns, int last_check, int par)
 {
     RUN_PROLOGUE();
     if (threadIdx.x != 0) return;
@@ -211,12 +306,13 @@ __global__ __launch_bounds__(kBlock) void k_solver_end(const RunDev *rds, const 
         int s = st->solve_max;
         st->sweeps_done = s;
         if (st->solve_mode == 0)
-            st->last_resid = s > 0 ? (double)__uint_as_float(R.sweep_max[s - 1]) : 0.0;
+            st->last_resid = s > 0 ? (double)read_sweep_max(R, s - 1) : 0.0;
         else
-            st->last_resid = __longlong_as_double((long long)R.resid[last_check]);
+            st->last_resid = read_resid(R, last_check);
+        st->phi_final = (st->phi_cur + par) & 1;
         st->solve_done = 1;
     }
-    if (st->solve_mode != 2) st->phi_cur = (st->phi_cur + st->sweeps_done) & 1;
+    if (st->solve_mode != 2) st->phi_cur = st->phi_final;
     st->sweeps_total += st->sweeps_done;
     st->solves += 1;
 }
@@ -269,16 +365,21 @@ void launch_solver_begin(const Launch &L, int use_args, int mode, double tol, in
 {
     LAUNCH_K(k_solver_begin, L, use_args, mode, tol, min_sweeps, max_sweeps, check_every);
 }
-void launch_jacobi_sweep(const Launch &L, int s, int mode, int check_slot, int first)
+void launch_jacobi_sweep(const Launch &L, int s) { LAUNCH_K(k_jacobi_sweep, L, s); }
+void launch_jacobi_block(const Launch &L, int s0, int nsw, int mode, int check_slot, int first,
+                         int par)
 {
-    LAUNCH_K(k_jacobi_sweep, L, s, mode, check_slot, first);
+    LAUNCH_K(k_jacobi_block, L, s0, nsw, mode, check_slot, first, par);
 }
-void launch_ir_update(const Launch &L, int s) { LAUNCH_K(k_ir_update, L, s); }
-void launch_residual(const Launch &L, int c, int s, int mode)
+void launch_ir_update(const Launch &L, int par) { LAUNCH_K(k_ir_update, L, par); }
+void launch_residual(const Launch &L, int c, int mode, int par)
 {
-    LAUNCH_K(k_residual, L, c, s, mode);
+    LAUNCH_K(k_residual, L, c, mode, par);
 }
-void launch_solver_end(const Launch &L, int last_check) { LAUNCH_K(k_solver_end, L, last_check); }
+void launch_solver_end(const Launch &L, int last_check, int par)
+{
+    LAUNCH_K(k_solver_end, L, last_check, par);
+}
 void launch_field(const Launch &L) { LAUNCH_K(k_field, L); }
 
 }  // namespace cupic

@@ -292,9 +292,112 @@ __global__ __launch_bounds__(kBlock) void k_half_kick(const RunDev *rds, const i
 
 // ---------------------------------------------------------------- push passes
 
+// Segmented inclusive prefix sum over the wave (segments are runs of lanes
+// that start at seg_start); lanes outside [seg_start, lane] do not add.
+__device__ __forceinline__ unsigned long long seg_scan_u64(unsigned long long v, int lane,
+                                                           int seg_start)
+{
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        unsigned long long u = __shfl_up(v, o, kWave);
+        if (lane - o >= seg_start) v += u;
+    }
+    return v;
+}
+
+// Adds one particle group's fixed-point charge (reduced sums, see
+// wave_deposit) to the four corners of tile cell key.
+__device__ __forceinline__ void tile_add(unsigned long long *tile, int TW, int key,
+                                         unsigned long long a1, unsigned long long a2,
+                                         long long sgn)
+{
+    long long sx = (long long)(a1 & ((1ull << 23) - 1));
+    long long sy = (long long)((a1 >> 23) & ((1ull << 23) - 1));
+    long long n = (long long)(a1 >> 46);
+    long long sxy = (long long)a2;
+    atomicAdd(&tile[key], (unsigned long long)(sgn * ((n << 32) - ((sx + sy) << 16) + sxy)));
+    atomicAdd(&tile[key + 1], (unsigned long long)(sgn * ((sx << 16) - sxy)));
+    atomicAdd(&tile[key + TW], (unsigned long long)(sgn * ((sy << 16) - sxy)));
+    atomicAdd(&tile[key + TW + 1], (unsigned long long)(sgn * sxy));
+}
+
+// LDS tile of E on the two node rows of a strip: columns col0 .. col0+nc.
+__device__ __forceinline__ void load_E_tile(const RunDev &R, float2 *Et, int j0, int col0,
+                                            int nc, int lane)
+{
+    int w = nc + 1;
+    for (int q = lane; q < 2 * w; q += kWave) {
+        int r = q / w, c = col0 + (q - r * w);
+        if (c >= R.ncx) c -= R.ncx;
+        Et[q] = R.E[(j0 + r) * R.ncx + c];
+    }
+}
+
+// The gather of push_one with the four nodes taken from the strip tile.
+__device__ __forceinline__ Pushed push_tile(const RunDev &R, const float2 *Et, int nc, int col0,
+                                            int j0, const float2 *__restrict__ E, float qdtm,
+                                            float x, float y, float vx, float vy, bool &in_strip)
+{
+    float fx, fy;
+    int oic, ojc;
+    cell_of(R, x, y, fx, fy, oic, ojc);
+    int cl = oic - col0;
+    in_strip = (ojc == j0) && cl >= 0 && cl < nc;
+    float ex, ey;
+    if (in_strip) {
+        int w = nc + 1;
+        float2 e00 = Et[cl], e10 = Et[cl + 1], e01 = Et[w + cl], e11 = Et[w + cl + 1];
+        float gx = 1.0f - fx, gy = 1.0f - fy;
+        float w00 = gx * gy, w10 = fx * gy, w01 = gx * fy, w11 = fx * fy;
+        float a = w00 * e00.x;
+        a = a + w10 * e10.x;
+        a = a + w01 * e01.x;
+        a = a + w11 * e11.x;
+        float b = w00 * e00.y;
+        b = b + w10 * e10.y;
+        b = b + w01 * e01.y;
+        b = b + w11 * e11.y;
+        ex = a;
+        ey = b;
+    } else {
+        gather(R, E, fx, fy, oic, ojc, ex, ey);
+    }
+    Pushed o;
+    float nvx = vx + qdtm * ex;
+    float nvy = vy + qdtm * ey;
+    float nx = x + R.dt_f * nvx;
+    float ny = y + R.dt_f * nvy;
+    if (nx < 0.0f) nx = nx + R.Lx_f;
+    if (nx >= R.Lx_f) nx = nx - R.Lx_f;
+    o.x = nx; o.y = ny; o.vx = nvx; o.vy = nvy;
+    o.oic = oic; o.ojc = ojc;
+    o.fx = 0.0f; o.fy = 0.0f; o.nic = 0; o.njc = 0; o.di = 0;
+    float ty = ny * R.inv_ds;
+    if (ty < 0.0f) { o.dir = kDirProbe; return o; }
+    if (ty >= (float)R.ncy) { o.dir = kDirTop; return o; }
+    int nic, njc;
+    cell_of(R, nx, ny, fx, fy, nic, njc);
+    o.fx = fx; o.fy = fy; o.nic = nic; o.njc = njc;
+    int di = nic - oic, dj = njc - ojc;
+    if (di == R.ncx - 1) di = -1;
+    else if (di == -(R.ncx - 1)) di = 1;
+    o.di = di;
+    if (di >= -1 && di <= 1 && dj >= -1 && dj <= 1)
+        o.dir = (di + 1) + 3 * (dj + 1);
+    else
+        o.dir = kDirFar;
+    return o;
+}
+
+// One wave processes the particles of a strip of whole old cells.  Pass 1
+// (kScatter false) counts arrivals per (old cell, direction) and appends
+// far movers; pass 2 replays the identical push and ranks with each counter
+// started at its slot base, stores every near particle at its sorted
+// position and deposits its charge.
 template <bool kScatter>
 __device__ __forceinline__ void push_task(const RunDev &R, RunState *st, const TaskInfo &T,
-                                          int *cntw, unsigned long long *tile, int lane)
+                                          int *cntw, float2 *Et, unsigned long long *tile,
+                                          int lane)
 {
     const int s = T.s, j0 = T.j0, col0 = T.col0, nc = T.nc;
     const int cur = st->cur;
@@ -305,32 +408,57 @@ __device__ __forceinline__ void push_task(const RunDev &R, RunState *st, const T
     const float *VX = R.pvx[s][cur], *VY = R.pvy[s][cur];
     float *OX = R.px[s][cur ^ 1], *OY = R.py[s][cur ^ 1];
     float *OVX = R.pvx[s][cur ^ 1], *OVY = R.pvy[s][cur ^ 1];
-    const int *base = R.base[s];
+    const int ncells = R.ncells;
     const float2 *E = R.E;
     const float qdtm = R.sp[s].qdtm;
     const long long sgn = s == 0 ? -1 : 1;
     const int TW = nc + 3;
-    for (int q = lane; q < nc * 9; q += kWave) cntw[q] = 0;
-    if (kScatter)
+    load_E_tile(R, Et, j0, col0, nc, lane);
+    if (kScatter) {
+        // counters start at the base of the slot each (cell, direction) fills
+        const int *base = R.base[s];
+        for (int q = lane; q < nc * 9; q += kWave) {
+            int cl = q / 9, d = q - 9 * cl;
+            int di = d % 3 - 1, dj = d / 3 - 1;
+            int jj = j0 + dj;
+            int v = 0;
+            if (jj >= 0 && jj < R.ncy) {
+                int ii = col0 + cl + di;
+                if (ii < 0) ii += R.ncx;
+                if (ii >= R.ncx) ii -= R.ncx;
+                v = base[(8 - d) * ncells + jj * R.ncx + ii];
+            }
+            cntw[q] = v;
+        }
         for (int q = lane; q < 4 * TW; q += kWave) tile[q] = 0ull;
+    } else {
+        for (int q = lane; q < nc * 9; q += kWave) cntw[q] = 0;
+    }
     wave_lds_sync();
     const unsigned long long lt = (1ull << lane) - 1ull;
     const unsigned long long upto = lt | (1ull << lane);
     unsigned long long probe_n = 0, top_n = 0;
     bool bad = false;
+    float nx_ = 0.0f, ny_ = 0.0f, nvx_ = 0.0f, nvy_ = 0.0f;
+    if (p0 + lane < p1) {
+        nx_ = X[p0 + lane]; ny_ = Y[p0 + lane]; nvx_ = VX[p0 + lane]; nvy_ = VY[p0 + lane];
+    }
     for (int i0 = p0; i0 < p1; i0 += kWave) {
         const int i = i0 + lane;
         const bool valid = i < p1;
-        float x = 0.0f, y = 0.0f, vx = 0.0f, vy = 0.0f;
-        if (valid) { x = X[i]; y = Y[i]; vx = VX[i]; vy = VY[i]; }
-        Pushed o = push_one(R, E, qdtm, x, y, vx, vy);
+        const float x = nx_, y = ny_, vx = nvx_, vy = nvy_;
+        if (i + kWave < p1) {  // prefetch the next chunk
+            nx_ = X[i + kWave]; ny_ = Y[i + kWave]; nvx_ = VX[i + kWave]; nvy_ = VY[i + kWave];
+        }
+        bool in_strip;
+        Pushed o = push_tile(R, Et, nc, col0, j0, E, qdtm, x, y, vx, vy, in_strip);
         int dir = valid ? o.dir : (int)kDirNone;
         int cl = o.oic - col0;
-        if (dir < 9 && (o.ojc != j0 || cl < 0 || cl >= nc)) {  // not in this strip
+        if (dir < 9 && !in_strip) {  // not in this strip: the sort invariant is broken
             dir = kDirFar;
             bad = true;
         }
-        // rank within (old cell, direction)
+        // rank within (old cell, direction); lanes of one old cell are contiguous
         unsigned long long mymask = 0;
 #pragma unroll
         for (int dd = 0; dd < 9; dd++) {
@@ -340,10 +468,10 @@ __device__ __forceinline__ void push_task(const RunDev &R, RunState *st, const T
         int clp = __shfl_up(cl, 1, kWave);
         bool head = valid && (lane == 0 || cl != clp);
         unsigned long long heads = __ballot(head);
+        int seg_start = valid ? 63 - __clzll((long long)(heads & upto)) : lane;
         int rank = 0;
         bool last = false;
         if (dir < 9) {
-            int seg_start = 63 - __clzll((long long)(heads & upto));
             unsigned long long after = heads & ~upto;
             unsigned long long hi = after ? ((1ull << (__ffsll((long long)after) - 1)) - 1ull) : ~0ull;
             unsigned long long seg = hi & ~((1ull << seg_start) - 1ull);
@@ -371,6 +499,7 @@ __device__ __forceinline__ void push_task(const RunDev &R, RunState *st, const T
                         rec.pad = 0;
                         rec.x = o.x; rec.y = o.y; rec.vx = o.vx; rec.vy = o.vy;
                         R.far[s][slot] = rec;
+                        atomicAdd(&R.far_cnt[s][rec.key], 1);
                     } else {
                         atomicOr(&st->err, kErrFarOverflow);
                     }
@@ -379,11 +508,9 @@ __device__ __forceinline__ void push_task(const RunDev &R, RunState *st, const T
             probe_n += __popcll(__ballot(dir == kDirProbe));
             top_n += __popcll(__ballot(dir == kDirTop));
         } else {
-            int dkey = -1;
             unsigned wx = 0, wy = 0;
             if (dir < 9) {
-                int key = o.njc * R.ncx + o.nic;
-                int dest = base[key * 10 + (8 - dir)] + rank;
+                int dest = rank;  // counters hold slot base + rank
                 if ((unsigned)dest < (unsigned)R.cap) {
                     OX[dest] = o.x;
                     OY[dest] = o.y;
@@ -393,15 +520,34 @@ __device__ __forceinline__ void push_task(const RunDev &R, RunState *st, const T
                     bad = true;
                 }
                 fixed_weights(o.fx, o.fy, wx, wy);
-                dkey = (o.njc - (j0 - 1)) * TW + (cl + o.di + 1);
             }
-            wave_deposit(tile, TW, dkey, wx, wy, sgn, lane);
+            // stayers: one segmented sum per old cell; movers: their own atomics
+            const bool stay = (dir == 4);
+            unsigned long long v1 = 0, v2 = 0;
+            if (dir < 9) {
+                v1 = (unsigned long long)wx | ((unsigned long long)wy << 23) | (1ull << 46);
+                v2 = (unsigned long long)wx * (unsigned long long)wy;
+            }
+            unsigned long long s1 = seg_scan_u64(stay ? v1 : 0ull, lane, seg_start);
+            unsigned long long s2 = seg_scan_u64(stay ? v2 : 0ull, lane, seg_start);
+            unsigned long long nxt_head = __ballot(head) & ~upto;
+            bool seg_last = valid && (nxt_head ? (__ffsll((long long)nxt_head) - 1 == lane + 1)
+                                               : (i + 1 >= p1 || lane == kWave - 1));
+            if (seg_last && (s1 >> 46) != 0ull && in_strip)
+                tile_add(tile, TW, 1 * TW + (cl + 1), s1, s2, sgn);
+            if (dir < 9 && !stay) {
+                int key = (o.njc - (j0 - 1)) * TW + (cl + o.di + 1);
+                tile_add(tile, TW, key, v1, v2, sgn);
+            }
         }
     }
     wave_lds_sync();
     if (!kScatter) {
-        int *cnt = R.cnt[s] + (size_t)c0 * 9;
-        for (int q = lane; q < nc * 9; q += kWave) cnt[q] = cntw[q];
+        int *cnt = R.cnt[s];
+        for (int q = lane; q < nc * 9; q += kWave) {
+            int cl = q / 9, d = q - 9 * cl;
+            cnt[d * ncells + c0 + cl] = cntw[q];
+        }
         if (lane == 0) {
             if (probe_n) atomicAdd(&st->probe[s], probe_n);
             if (top_n) atomicAdd(&st->top[s], top_n);
@@ -418,9 +564,11 @@ __global__ __launch_bounds__(kBlock) void k_push_rank(const RunDev *rds, const i
 {
     RUN_PROLOGUE();
     __shared__ int cnts[kBlock / kWave][kCmax * 9];
+    __shared__ float2 etiles[kBlock / kWave][2 * (kCmax + 1)];
     TaskInfo T = task_of(R, local);
     if (!T.ok) return;
-    push_task<false>(R, st, T, cnts[threadIdx.x >> 6], nullptr, threadIdx.x & (kWave - 1));
+    int w = threadIdx.x >> 6;
+    push_task<false>(R, st, T, cnts[w], etiles[w], nullptr, threadIdx.x & (kWave - 1));
 }
 
 __global__ __launch_bounds__(kBlock) void k_scatter(const RunDev *rds, const int *start,
@@ -428,11 +576,12 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const RunDev *rds, const int
 {
     RUN_PROLOGUE();
     __shared__ int cnts[kBlock / kWave][kCmax * 9];
+    __shared__ float2 etiles[kBlock / kWave][2 * (kCmax + 1)];
     __shared__ unsigned long long tiles[kBlock / kWave][4 * kTileW];
     TaskInfo T = task_of(R, local);
     if (!T.ok) return;
-    push_task<true>(R, st, T, cnts[threadIdx.x >> 6], tiles[threadIdx.x >> 6],
-                    threadIdx.x & (kWave - 1));
+    int w = threadIdx.x >> 6;
+    push_task<true>(R, st, T, cnts[w], etiles[w], tiles[w], threadIdx.x & (kWave - 1));
 }
 
 // ---------------------------------------------------------------- launchers
