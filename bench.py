@@ -73,11 +73,26 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-timing", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="first copy this many MiB device-to-device with torch: a plain copy "
+                         "kernel of known size that scales profiler byte counters")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: a single run of this path lives on one device "
               "(multi-GPU sharding is a design note, not part of this build)", file=sys.stderr)
         sys.exit(2)
+
+    if a.copy_probe_mib > 0:
+        import torch
+        n = a.copy_probe_mib * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"# copy probe: {a.copy_probe_mib} MiB read + {a.copy_probe_mib} MiB written",
+              file=sys.stderr, flush=True)
+        del src, dst
 
     import cupic_amd as cp
     name, cus = cp.init_dev(0)

@@ -130,6 +130,32 @@ struct RunDev {
 
 // ---------------------------------------------------------------- helpers
 
+// Pointers held in RunDev are generic; casting them to the global address
+// space lets the compiler emit global (not flat) memory instructions, whose
+// completion is tracked apart from LDS traffic.
+#define CUPIC_GLOBAL __attribute__((address_space(1)))
+
+template <class T>
+__device__ __forceinline__ const CUPIC_GLOBAL T *gp(const T *p)
+{
+    return (const CUPIC_GLOBAL T *)p;
+}
+
+template <class T>
+__device__ __forceinline__ CUPIC_GLOBAL T *gp(T *p)
+{
+    return (CUPIC_GLOBAL T *)p;
+}
+
+// float2 is a class type that cannot be copied through an address-space
+// reference; load it as a native two-element vector.
+__device__ __forceinline__ float2 ld_f2(const float2 *p, int k)
+{
+    typedef float v2f __attribute__((ext_vector_type(2)));
+    v2f v = ((const CUPIC_GLOBAL v2f *)p)[k];
+    return make_float2(v.x, v.y);
+}
+
 __device__ __forceinline__ int find_run(const int *__restrict__ start, int nruns, int b)
 {
     int lo = 0, hi = nruns - 1;
@@ -204,9 +230,30 @@ __device__ __forceinline__ double u01_open(unsigned long long r)
 
 // ---------------------------------------------------------------- cell keys
 
+// Geometry a particle kernel needs, copied once into registers: reading it
+// through the RunDev reference inside a loop that stores particles would
+// reload it after every store (the compiler cannot rule out aliasing).
+struct Geo {
+    int ncx, ncy;
+    float inv_ds, Lx_f, Ly_f, dt_f;
+};
+
+__device__ __forceinline__ Geo geo_of(const RunDev &R)
+{
+    Geo g;
+    g.ncx = R.ncx;
+    g.ncy = R.ncy;
+    g.inv_ds = R.inv_ds;
+    g.Lx_f = R.Lx_f;
+    g.Ly_f = R.Ly_f;
+    g.dt_f = R.dt_f;
+    return g;
+}
+
 // Cell of an FP32 position and its in-cell fractions: one product, one
 // floor and clamps, evaluated without contraction.
-__device__ __forceinline__ void cell_of(const RunDev &R, float x, float y, float &fx, float &fy,
+template <class G>
+__device__ __forceinline__ void cell_of(const G &R, float x, float y, float &fx, float &fy,
                                         int &ic, int &jc)
 {
     float tx = x * R.inv_ds;
@@ -220,13 +267,14 @@ __device__ __forceinline__ void cell_of(const RunDev &R, float x, float y, float
 }
 
 // CIC gather from interleaved (Ex, Ey), fixed operation order.
-__device__ __forceinline__ void gather(const RunDev &R, const float2 *__restrict__ E, float fx,
+template <class G>
+__device__ __forceinline__ void gather(const G &R, const float2 *__restrict__ Eg, float fx,
                                        float fy, int ic, int jc, float &ex, float &ey)
 {
     int i1 = (ic + 1 == R.ncx) ? 0 : ic + 1;
     int k00 = jc * R.ncx + ic, k10 = jc * R.ncx + i1;
     int k01 = k00 + R.ncx, k11 = k10 + R.ncx;
-    float2 e00 = E[k00], e10 = E[k10], e01 = E[k01], e11 = E[k11];
+    float2 e00 = ld_f2(Eg, k00), e10 = ld_f2(Eg, k10), e01 = ld_f2(Eg, k01), e11 = ld_f2(Eg, k11);
     float gx = 1.0f - fx, gy = 1.0f - fy;
     float w00 = gx * gy, w10 = fx * gy, w01 = gx * fy, w11 = fx * fy;
     float a = w00 * e00.x;
@@ -253,7 +301,8 @@ struct Pushed {
 // Leapfrog push of one particle (src/particles.cu:110-231 semantics),
 // periodic wrap into [0, Lx) (src/cc.cu:775-810, Q6), absorption at y < 0
 // (probe) and y >= Ly (top) (src/cc.cu:182-185), new cell and direction.
-__device__ __forceinline__ Pushed push_one(const RunDev &R, const float2 *__restrict__ E,
+template <class G>
+__device__ __forceinline__ Pushed push_one(const G &R, const float2 *__restrict__ E,
                                            float qdtm, float x, float y, float vx, float vy)
 {
     Pushed o;

@@ -154,10 +154,10 @@ __global__ __launch_bounds__(kBlock) void k_jacobi_sweep(const RunDev *rds, cons
             int kw = (i == 0) ? k + ncx - 1 : k - 1;
             int ke = (i == ncx - 1) ? k - ncx + 1 : k + 1;
             int pc = st->phi_cur;
-            const float *in = R.phi32[(pc + s) & 1];
-            float *out = R.phi32[(pc + s + 1) & 1];
+            const CUPIC_GLOBAL float *in = gp(R.phi32[(pc + s) & 1]);
+            CUPIC_GLOBAL float *out = gp(R.phi32[(pc + s + 1) & 1]);
             float c = in[k];
-            float v = 0.25f * ((((R.b32[k] + in[kw]) + in[ke]) + in[k + ncx]) + in[k - ncx]);
+            float v = 0.25f * ((((gp(R.b32)[k] + in[kw]) + in[ke]) + in[k + ncx]) + in[k - ncx]);
             out[k] = v;
             dmax = fabsf(v - c);
         }
@@ -196,16 +196,16 @@ __global__ __launch_bounds__(kBlock) void k_jacobi_block(const RunDev *rds, cons
     const int tiles_x = (ncx + kJacTX - 1) / kJacTX;
     const int by = local / tiles_x, bx = local - by * tiles_x;
     const int gx0 = bx * kJacTX - kJacH, gy0 = 1 + by * kJacTY - kJacH;
-    const float *in, *rhs;
-    float *out;
+    const CUPIC_GLOBAL float *in, *rhs;
+    CUPIC_GLOBAL float *out;
     if (mode == 2) {
-        in = R.d32[par & 1];
-        out = R.d32[(par + 1) & 1];
-        rhs = R.r32;
+        in = gp(R.d32[par & 1]);
+        out = gp(R.d32[(par + 1) & 1]);
+        rhs = gp(R.r32);
     } else {
-        in = R.phi32[(st->phi_cur + par) & 1];
-        out = R.phi32[(st->phi_cur + par + 1) & 1];
-        rhs = R.b32;
+        in = gp(R.phi32[(st->phi_cur + par) & 1]);
+        out = gp(R.phi32[(st->phi_cur + par + 1) & 1]);
+        rhs = gp(R.b32);
     }
     for (int q = threadIdx.x; q < LN; q += kBlock) {
         int lr = q / LW, lc = q - lr * LW;
@@ -279,15 +279,15 @@ __global__ __launch_bounds__(kBlock) void k_residual(const RunDev *rds, const in
             int ke = (i == ncx - 1) ? k - ncx + 1 : k + 1;
             double pc_, pw, pe, pn, ps;
             if (mode == 2) {
-                const double *p = R.phi64;
+                const CUPIC_GLOBAL double *p = gp(R.phi64);
                 pc_ = p[k]; pw = p[kw]; pe = p[ke]; pn = p[k + ncx]; ps = p[k - ncx];
             } else {
-                const float *p = R.phi32[(st->phi_cur + par) & 1];
+                const CUPIC_GLOBAL float *p = gp(R.phi32[(st->phi_cur + par) & 1]);
                 pc_ = p[k]; pw = p[kw]; pe = p[ke]; pn = p[k + ncx]; ps = p[k - ncx];
             }
-            double sum = (((R.b64[k] + pw) + pe) + pn) + ps;
+            double sum = (((gp(R.b64)[k] + pw) + pe) + pn) + ps;
             double rr = sum - 4.0 * pc_;
-            if (mode == 2) R.r32[k] = (float)rr;
+            if (mode == 2) gp(R.r32)[k] = (float)rr;
             rmax = fabs(0.25 * rr);
         }
     }

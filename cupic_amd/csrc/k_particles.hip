@@ -196,7 +196,7 @@ __device__ __forceinline__ void flush_tile(const RunDev &R, const unsigned long 
         if (col < 0) col += R.ncx;
         if (col >= R.ncx) col -= R.ncx;
         if (row < 0 || row > R.ncy) continue;
-        atomicAdd(&R.acc[row * R.ncx + col], v);
+        atomicAdd(&R.acc[row * R.ncx + col], v);  // global: acc lives in HBM
     }
 }
 
@@ -329,12 +329,12 @@ __device__ __forceinline__ void load_E_tile(const RunDev &R, float2 *Et, int j0,
     for (int q = lane; q < 2 * w; q += kWave) {
         int r = q / w, c = col0 + (q - r * w);
         if (c >= R.ncx) c -= R.ncx;
-        Et[q] = R.E[(j0 + r) * R.ncx + c];
+        Et[q] = ld_f2(R.E, (j0 + r) * R.ncx + c);
     }
 }
 
 // The gather of push_one with the four nodes taken from the strip tile.
-__device__ __forceinline__ Pushed push_tile(const RunDev &R, const float2 *Et, int nc, int col0,
+__device__ __forceinline__ Pushed push_tile(const Geo &R, const float2 *Et, int nc, int col0,
                                             int j0, const float2 *__restrict__ E, float qdtm,
                                             float x, float y, float vx, float vy, bool &in_strip)
 {
@@ -343,25 +343,23 @@ __device__ __forceinline__ Pushed push_tile(const RunDev &R, const float2 *Et, i
     cell_of(R, x, y, fx, fy, oic, ojc);
     int cl = oic - col0;
     in_strip = (ojc == j0) && cl >= 0 && cl < nc;
-    float ex, ey;
-    if (in_strip) {
-        int w = nc + 1;
-        float2 e00 = Et[cl], e10 = Et[cl + 1], e01 = Et[w + cl], e11 = Et[w + cl + 1];
-        float gx = 1.0f - fx, gy = 1.0f - fy;
-        float w00 = gx * gy, w10 = fx * gy, w01 = gx * fy, w11 = fx * fy;
-        float a = w00 * e00.x;
-        a = a + w10 * e10.x;
-        a = a + w01 * e01.x;
-        a = a + w11 * e11.x;
-        float b = w00 * e00.y;
-        b = b + w10 * e10.y;
-        b = b + w01 * e01.y;
-        b = b + w11 * e11.y;
-        ex = a;
-        ey = b;
-    } else {
-        gather(R, E, fx, fy, oic, ojc, ex, ey);
-    }
+    // A particle outside the strip means the sort invariant is broken: it is
+    // reported (kErrSortInvariant) and reads a clamped tile entry instead of
+    // a global load, which would stall the prefetch of every chunk.
+    cl = min(max(cl, 0), nc - 1);
+    int w = nc + 1;
+    float2 e00 = Et[cl], e10 = Et[cl + 1], e01 = Et[w + cl], e11 = Et[w + cl + 1];
+    float gx = 1.0f - fx, gy = 1.0f - fy;
+    float w00 = gx * gy, w10 = fx * gy, w01 = gx * fy, w11 = fx * fy;
+    float ex = w00 * e00.x;
+    ex = ex + w10 * e10.x;
+    ex = ex + w01 * e01.x;
+    ex = ex + w11 * e11.x;
+    float ey = w00 * e00.y;
+    ey = ey + w10 * e10.y;
+    ey = ey + w01 * e01.y;
+    ey = ey + w11 * e11.y;
+    (void)E;
     Pushed o;
     float nvx = vx + qdtm * ex;
     float nvy = vy + qdtm * ey;
@@ -400,14 +398,18 @@ __device__ __forceinline__ void push_task(const RunDev &R, RunState *st, const T
                                           int lane)
 {
     const int s = T.s, j0 = T.j0, col0 = T.col0, nc = T.nc;
+    const Geo G = geo_of(R);
+    const int cap = R.cap, far_cap = R.far_cap;
+    int *const far_cnt = R.far_cnt[s];
+    FarRec *const farl = R.far[s];
     const int cur = st->cur;
-    const int *off = R.off[s][cur];
+    const CUPIC_GLOBAL int *off = gp(R.off[s][cur]);
     const int c0 = j0 * R.ncx + col0;
     const int p0 = off[c0], p1 = off[c0 + nc];
-    const float *X = R.px[s][cur], *Y = R.py[s][cur];
-    const float *VX = R.pvx[s][cur], *VY = R.pvy[s][cur];
-    float *OX = R.px[s][cur ^ 1], *OY = R.py[s][cur ^ 1];
-    float *OVX = R.pvx[s][cur ^ 1], *OVY = R.pvy[s][cur ^ 1];
+    const CUPIC_GLOBAL float *X = gp(R.px[s][cur]), *Y = gp(R.py[s][cur]);
+    const CUPIC_GLOBAL float *VX = gp(R.pvx[s][cur]), *VY = gp(R.pvy[s][cur]);
+    CUPIC_GLOBAL float *OX = gp(R.px[s][cur ^ 1]), *OY = gp(R.py[s][cur ^ 1]);
+    CUPIC_GLOBAL float *OVX = gp(R.pvx[s][cur ^ 1]), *OVY = gp(R.pvy[s][cur ^ 1]);
     const int ncells = R.ncells;
     const float2 *E = R.E;
     const float qdtm = R.sp[s].qdtm;
@@ -416,7 +418,7 @@ __device__ __forceinline__ void push_task(const RunDev &R, RunState *st, const T
     load_E_tile(R, Et, j0, col0, nc, lane);
     if (kScatter) {
         // counters start at the base of the slot each (cell, direction) fills
-        const int *base = R.base[s];
+        const CUPIC_GLOBAL int *base = gp(R.base[s]);
         for (int q = lane; q < nc * 9; q += kWave) {
             int cl = q / 9, d = q - 9 * cl;
             int di = d % 3 - 1, dj = d / 3 - 1;
@@ -451,7 +453,7 @@ __device__ __forceinline__ void push_task(const RunDev &R, RunState *st, const T
             nx_ = X[i + kWave]; ny_ = Y[i + kWave]; nvx_ = VX[i + kWave]; nvy_ = VY[i + kWave];
         }
         bool in_strip;
-        Pushed o = push_tile(R, Et, nc, col0, j0, E, qdtm, x, y, vx, vy, in_strip);
+        Pushed o = push_tile(G, Et, nc, col0, j0, E, qdtm, x, y, vx, vy, in_strip);
         int dir = valid ? o.dir : (int)kDirNone;
         int cl = o.oic - col0;
         if (dir < 9 && !in_strip) {  // not in this strip: the sort invariant is broken
@@ -491,15 +493,15 @@ __device__ __forceinline__ void push_task(const RunDev &R, RunState *st, const T
                 fb = __shfl(fb, leader, kWave);
                 if (dir == kDirFar) {
                     int slot = fb + __popcll(fm & lt);
-                    if (slot < R.far_cap) {
-                        FarRec rec;
-                        rec.key = o.njc * R.ncx + o.nic;
-                        rec.idx = i;
-                        rec.rank = 0;
-                        rec.pad = 0;
-                        rec.x = o.x; rec.y = o.y; rec.vx = o.vx; rec.vy = o.vy;
-                        R.far[s][slot] = rec;
-                        atomicAdd(&R.far_cnt[s][rec.key], 1);
+                    if (slot < far_cap) {
+                        CUPIC_GLOBAL FarRec *f = gp(farl) + slot;
+                        int key = o.njc * G.ncx + o.nic;
+                        f->key = key;
+                        f->idx = i;
+                        f->rank = 0;
+                        f->pad = 0;
+                        f->x = o.x; f->y = o.y; f->vx = o.vx; f->vy = o.vy;
+                        atomicAdd(&far_cnt[key], 1);
                     } else {
                         atomicOr(&st->err, kErrFarOverflow);
                     }
@@ -511,7 +513,7 @@ __device__ __forceinline__ void push_task(const RunDev &R, RunState *st, const T
             unsigned wx = 0, wy = 0;
             if (dir < 9) {
                 int dest = rank;  // counters hold slot base + rank
-                if ((unsigned)dest < (unsigned)R.cap) {
+                if ((unsigned)dest < (unsigned)cap) {
                     OX[dest] = o.x;
                     OY[dest] = o.y;
                     OVX[dest] = o.vx;
@@ -543,7 +545,7 @@ __device__ __forceinline__ void push_task(const RunDev &R, RunState *st, const T
     }
     wave_lds_sync();
     if (!kScatter) {
-        int *cnt = R.cnt[s];
+        CUPIC_GLOBAL int *cnt = gp(R.cnt[s]);
         for (int q = lane; q < nc * 9; q += kWave) {
             int cl = q / 9, d = q - 9 * cl;
             cnt[d * ncells + c0 + cl] = cntw[q];
