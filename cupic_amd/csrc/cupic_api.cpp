@@ -195,18 +195,37 @@ void seq_solve(const Target &T, int use_args, int mode, double tol, int min_swee
     }
     Launch B = T.L(kShapeJacBlk);
     if (check < 1) check = 1;
+    const int per = kJacH - 1;  // sweeps per blocked launch
     int c = 0, s = 0, nlaunch = 0, dpar = 0;
+    if (mode == 1) {
+        // FP64 residuals: the initial one here, the others fused into the
+        // last launch of each block of check sweeps
+        launch_residual(N, 0, mode, 0);
+        while (s < max_sweeps) {
+            int k = std::min(check, max_sweeps - s);
+            for (int q = 0; q < k;) {
+                int nsw = std::min(per, k - q);
+                bool last = q + nsw >= k;
+                launch_jacobi_block(B, s + q, nsw, 1, q == 0 ? c : -1, 0, nlaunch & 1,
+                                    last ? c + 1 : -1);
+                nlaunch++;
+                q += nsw;
+            }
+            s += k;
+            c += 1;
+        }
+        launch_solver_end(T.L(kShapeRun), c, nlaunch & 1);
+        return;
+    }
     for (;;) {
-        if (mode == 2 && c > 0) launch_ir_update(N, dpar);
-        launch_residual(N, c, mode, nlaunch & 1);
+        if (c > 0) launch_ir_update(N, dpar);
+        launch_residual(N, c, mode, 0);
         if (s >= max_sweeps) break;
         int k = std::min(check, max_sweeps - s);
         int dl = 0;
         for (int q = 0; q < k;) {
-            int nsw = std::min(kJacH, k - q);
-            launch_jacobi_block(B, s + q, nsw, mode, q == 0 ? c : -1,
-                                (mode == 2 && q == 0) ? 1 : 0, mode == 2 ? (dl & 1) : (nlaunch & 1));
-            nlaunch++;
+            int nsw = std::min(per, k - q);
+            launch_jacobi_block(B, s + q, nsw, 2, q == 0 ? c : -1, q == 0 ? 1 : 0, dl & 1, -1);
             dl++;
             q += nsw;
         }

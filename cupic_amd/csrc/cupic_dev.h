@@ -28,9 +28,10 @@ constexpr int kCmax = 16;              // max cells per wave task
 constexpr int kTileW = kCmax + 3;      // deposit tile width (node columns)
 constexpr int kScanCells = 1024;       // cells per scan block
 constexpr int kRedSlots = 16;         // atomic slots per reduced quantity
-constexpr int kJacTX = 64;             // blocked Jacobi tile (x)
-constexpr int kJacTY = 16;             // blocked Jacobi tile (y)
-constexpr int kJacH = 10;              // sweeps per blocked launch = LDS halo
+constexpr int kJacRows = 16;           // blocked Jacobi: rows per wave
+constexpr int kJacH = 11;              // halo; at most kJacH - 1 sweeps per launch
+constexpr int kJacTX = 64 - 2 * kJacH;                   // tile columns (42)
+constexpr int kJacTY = kJacRows * (256 / 64) - 2 * kJacH;  // tile rows (42)
 
 enum Dir : int { kDirFar = 9, kDirProbe = 10, kDirTop = 11, kDirNone = 15 };
 

@@ -168,21 +168,27 @@ __global__ __launch_bounds__(kBlock) void k_jacobi_sweep(const RunDev *rds, cons
     });
 }
 
-// nsw <= kJacH Jacobi sweeps s0 .. s0+nsw-1 in one launch (modes 1, 2).
-// Each workgroup loads its kJacTX x kJacTY tile with a kJacH-node halo into
-// LDS, sweeps there (the valid region shrinks by one node per sweep and
-// still covers the tile after nsw sweeps) and writes the tile back: the
-// iterates equal those of nsw single sweeps.  Mode 1 relaxes phi (rhs b),
-// mode 2 the refinement correction d (rhs r), d starting at 0 when first.
-// par selects the input buffer; the output goes to the other one.
+// nsw <= kJacH - 1 Jacobi sweeps s0 .. s0+nsw-1 in one launch (modes 1, 2).
+// A workgroup holds a 64 x (4 * kJacRows) node region -- a kJacTX x kJacTY
+// tile with a kJacH-node halo -- in registers: lane = region column, each
+// wave kJacRows consecutive rows.  East/west neighbours come from the next
+// lanes, north/south from the thread's own rows; only the rows at wave
+// boundaries go through LDS.  The valid region shrinks by one node per
+// sweep and still covers the tile and one ring after nsw sweeps, so the
+// iterates equal those of nsw single sweeps (same operations per node).
+// Mode 1 relaxes phi (rhs b), mode 2 the refinement correction d (rhs r),
+// d starting at 0 when first.  par selects the input buffer; the output goes
+// to the other one.  With resid_slot >= 0 (mode 1) the launch also reduces
+// the FP64 residual of its tile after the last sweep.
 __global__ __launch_bounds__(kBlock) void k_jacobi_block(const RunDev *rds, const int *start,
                                                          int nruns, int s0, int nsw, int mode,
-                                                         int check_slot, int first, int par)
+                                                         int check_slot, int first, int par,
+                                                         int resid_slot)
 {
     RUN_PROLOGUE();
-    constexpr int LW = kJacTX + 2 * kJacH, LH = kJacTY + 2 * kJacH, LN = LW * LH;
-    __shared__ float A[2][LN];
-    __shared__ float Bv[LN];
+    constexpr int NR = kJacRows;
+    __shared__ float xch[2][kBlock / kWave][2][kWave];
+    __shared__ double shr[kBlock / kWave];
     if (st->solve_done) return;
     if (check_slot >= 0) {
         double res = read_resid(R, check_slot);
@@ -193,9 +199,12 @@ __global__ __launch_bounds__(kBlock) void k_jacobi_block(const RunDev *rds, cons
     }
     if (s0 >= st->solve_max) return;
     const int ncx = R.ncx, ncy = R.ncy;
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x >> 6;
     const int tiles_x = (ncx + kJacTX - 1) / kJacTX;
     const int by = local / tiles_x, bx = local - by * tiles_x;
-    const int gx0 = bx * kJacTX - kJacH, gy0 = 1 + by * kJacTY - kJacH;
+    int gx = (bx * kJacTX - kJacH + lane) % ncx;
+    if (gx < 0) gx += ncx;
+    const int row0 = 1 + by * kJacTY - kJacH + NR * w;  // global row of v[0]
     const CUPIC_GLOBAL float *in, *rhs;
     CUPIC_GLOBAL float *out;
     if (mode == 2) {
@@ -207,41 +216,76 @@ __global__ __launch_bounds__(kBlock) void k_jacobi_block(const RunDev *rds, cons
         out = gp(R.phi32[(st->phi_cur + par + 1) & 1]);
         rhs = gp(R.b32);
     }
-    for (int q = threadIdx.x; q < LN; q += kBlock) {
-        int lr = q / LW, lc = q - lr * LW;
-        int gy = gy0 + lr;
-        int gx = (gx0 + lc) % ncx;
-        if (gx < 0) gx += ncx;
-        float a = 0.0f, bb = 0.0f;
-        if (gy >= 0 && gy <= ncy) {
-            int g = gy * ncx + gx;
-            a = (mode == 2 && first) ? 0.0f : in[g];
-            bb = rhs[g];
-        }
-        A[0][q] = a;
-        A[1][q] = a;
-        Bv[q] = bb;
+    float v[NR], b[NR];
+    unsigned upd = 0;  // bit i: row i of this thread is updated
+    const bool lane_in = lane >= 1 && lane <= kWave - 2;
+    const bool zero_in = (mode == 2 && first);
+#pragma unroll
+    for (int i = 0; i < NR; i++) {
+        int gy = row0 + i;
+        int lr = NR * w + i;
+        int gyc = min(max(gy, 0), ncy);  // rows outside the grid load a clamped row
+        unsigned g = (unsigned)(gyc * ncx + gx);
+        float vin = in[g], bin = rhs[g];
+        bool ok = gy >= 0 && gy <= ncy;
+        v[i] = (ok && !zero_in) ? vin : 0.0f;
+        b[i] = ok ? bin : 0.0f;
+        if (lane_in && lr >= 1 && lr <= NR * (kBlock / kWave) - 2 && gy >= 1 && gy < ncy)
+            upd |= 1u << i;
     }
-    __syncthreads();
-    int cur = 0;
+#pragma unroll 1
     for (int j = 0; j < nsw; j++) {
-        const float *a = A[cur];
-        float *o = A[cur ^ 1];
-        for (int q = threadIdx.x; q < LN; q += kBlock) {
-            int lr = q / LW, lc = q - lr * LW;
-            int gy = gy0 + lr;
-            if (lr >= 1 && lr < LH - 1 && lc >= 1 && lc < LW - 1 && gy >= 1 && gy < ncy)
-                o[q] = 0.25f * ((((Bv[q] + a[q - 1]) + a[q + 1]) + a[q + LW]) + a[q - LW]);
-        }
+        float(*x)[2][kWave] = xch[j & 1];
+        x[w][0][lane] = v[0];
+        x[w][1][lane] = v[NR - 1];
         __syncthreads();
-        cur ^= 1;
+        float prev = w > 0 ? x[w - 1][1][lane] : 0.0f;  // old value of the row below
+        const float above = w < kBlock / kWave - 1 ? x[w + 1][0][lane] : 0.0f;
+#pragma unroll
+        for (int i = 0; i < NR; i++) {
+            const float cur = v[i];
+            float W = __shfl_up(cur, 1, kWave);
+            float E = __shfl_down(cur, 1, kWave);
+            float N = i < NR - 1 ? v[i + 1] : above;
+            float nvv = 0.25f * ((((b[i] + W) + E) + N) + prev);
+            prev = cur;
+            v[i] = (upd >> i) & 1u ? nvv : cur;
+        }
     }
-    for (int q = threadIdx.x; q < kJacTX * kJacTY; q += kBlock) {
-        int tr = q / kJacTX, tc = q - tr * kJacTX;
-        int gy = 1 + by * kJacTY + tr, gx = bx * kJacTX + tc;
-        if (gy < ncy && gx < ncx)
-            out[gy * ncx + gx] = A[cur][(tr + kJacH) * LW + (tc + kJacH)];
+    const bool col_in = lane >= kJacH && lane < kJacH + kJacTX && bx * kJacTX + lane - kJacH < ncx;
+#pragma unroll
+    for (int i = 0; i < NR; i++) {
+        int lr = NR * w + i, gy = row0 + i;
+        if (col_in && lr >= kJacH && lr < kJacH + kJacTY && gy >= 1 && gy < ncy)
+            out[gy * ncx + (bx * kJacTX + lane - kJacH)] = v[i];
     }
+    if (resid_slot < 0) return;
+    // FP64 residual of the tile's interior nodes after the last sweep
+    float(*x)[2][kWave] = xch[nsw & 1];
+    x[w][0][lane] = v[0];
+    x[w][1][lane] = v[NR - 1];
+    __syncthreads();
+    const float below = w > 0 ? x[w - 1][1][lane] : 0.0f;
+    const float above = w < kBlock / kWave - 1 ? x[w + 1][0][lane] : 0.0f;
+    const CUPIC_GLOBAL double *b64 = gp(R.b64);
+    double rmax = 0.0;
+#pragma unroll
+    for (int i = 0; i < NR; i++) {
+        float W = __shfl_up(v[i], 1, kWave);
+        float E = __shfl_down(v[i], 1, kWave);
+        float S = i > 0 ? v[i - 1] : below;
+        float N = i < NR - 1 ? v[i + 1] : above;
+        int lr = NR * w + i, gy = row0 + i;
+        if (col_in && lr >= kJacH && lr < kJacH + kJacTY && gy >= 1 && gy < ncy) {
+            double sum = (((b64[gy * ncx + (bx * kJacTX + lane - kJacH)] + (double)W) + (double)E) +
+                          (double)N) + (double)S;
+            rmax = fmax(rmax, fabs(0.25 * (sum - 4.0 * (double)v[i])));
+        }
+    }
+    unsigned long long *slot = &R.resid[resid_slot * kRedSlots + (local & (kRedSlots - 1))];
+    block_max_atomic(rmax, shr, [&](double m) {
+        if (m > 0.0) atomicMax(slot, (unsigned long long)__double_as_longlong(m));
+    });
 }
 
 // mode 2: phi64 += d, the correction solved by the last cycle of sweeps
@@ -367,9 +411,9 @@ void launch_solver_begin(const Launch &L, int use_args, int mode, double tol, in
 }
 void launch_jacobi_sweep(const Launch &L, int s) { LAUNCH_K(k_jacobi_sweep, L, s); }
 void launch_jacobi_block(const Launch &L, int s0, int nsw, int mode, int check_slot, int first,
-                         int par)
+                         int par, int resid_slot)
 {
-    LAUNCH_K(k_jacobi_block, L, s0, nsw, mode, check_slot, first, par);
+    LAUNCH_K(k_jacobi_block, L, s0, nsw, mode, check_slot, first, par, resid_slot);
 }
 void launch_ir_update(const Launch &L, int par) { LAUNCH_K(k_ir_update, L, par); }
 void launch_residual(const Launch &L, int c, int mode, int par)
