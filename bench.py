@@ -92,6 +92,20 @@ def main():
         torch.cuda.synchronize()
         print(f"# copy probe: {a.copy_probe_mib} MiB read + {a.copy_probe_mib} MiB written",
               file=sys.stderr, flush=True)
+        # achievable HBM bandwidth of plain streaming kernels, for reference
+        nb = n * 4
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        for label, fn, nbytes in (("copy", lambda: dst.copy_(src), 2 * nb),
+                                  ("read (sum)", lambda: src.sum(), nb),
+                                  ("write (fill)", lambda: dst.fill_(2.0), nb)):
+            fn()
+            ev[0].record()
+            for _ in range(5):
+                fn()
+            ev[1].record()
+            torch.cuda.synchronize()
+            ms = ev[0].elapsed_time(ev[1]) / 5
+            print(f"# {label}: {nbytes / ms / 1e6:.0f} GB/s", file=sys.stderr, flush=True)
         del src, dst
 
     import cupic_amd as cp

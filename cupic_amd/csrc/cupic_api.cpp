@@ -588,10 +588,7 @@ int cupic_run_create(const cupic_params *pin, int64_t capacity_hint, cupic_run *
     int st = CUPIC_OK;
     for (int s = 0; s < 2 && !st; s++) {
         for (int b = 0; b < 2; b++) {
-            h.px[s][b] = dev_alloc<float>(r, z.cap, &st);
-            h.py[s][b] = dev_alloc<float>(r, z.cap, &st);
-            h.pvx[s][b] = dev_alloc<float>(r, z.cap, &st);
-            h.pvy[s][b] = dev_alloc<float>(r, z.cap, &st);
+            h.part[s][b] = dev_alloc<float4>(r, z.cap, &st);
             h.off[s][b] = dev_alloc<int>(r, z.ncells + 1, &st);
         }
         h.cnt[s] = dev_alloc<int>(r, z.ncells * 9, &st);
@@ -869,12 +866,15 @@ int cupic_get_particles(cupic_run *r, int species, float *x, float *y, float *vx
     int64_t N = s.N[species];
     if (n) *n = N;
     if (capacity < N) return set_err(&r->err, CUPIC_ERR_ARG, "host buffer too small");
-    int c = s.cur;
-    size_t b = (size_t)N * sizeof(float);
-    if (x) HIPTRY(&r->err, hipMemcpy(x, r->h.px[species][c], b, hipMemcpyDeviceToHost));
-    if (y) HIPTRY(&r->err, hipMemcpy(y, r->h.py[species][c], b, hipMemcpyDeviceToHost));
-    if (vx) HIPTRY(&r->err, hipMemcpy(vx, r->h.pvx[species][c], b, hipMemcpyDeviceToHost));
-    if (vy) HIPTRY(&r->err, hipMemcpy(vy, r->h.pvy[species][c], b, hipMemcpyDeviceToHost));
+    std::vector<float4> buf((size_t)N);
+    HIPTRY(&r->err, hipMemcpy(buf.data(), r->h.part[species][s.cur], (size_t)N * sizeof(float4),
+                              hipMemcpyDeviceToHost));
+    for (int64_t k = 0; k < N; k++) {
+        if (x) x[k] = buf[k].x;
+        if (y) y[k] = buf[k].y;
+        if (vx) vx[k] = buf[k].z;
+        if (vy) vy[k] = buf[k].w;
+    }
     return st;
 }
 
@@ -897,18 +897,14 @@ int cupic_set_particles(cupic_run *r, int species, const float *x, const float *
     }
     for (int k = 0; k < h.ncells; k++) cnt[k + 1] += cnt[k];
     std::vector<int> off(cnt);
-    std::vector<float> sx((size_t)n), sy((size_t)n), svx((size_t)n), svy((size_t)n);
+    std::vector<float4> buf((size_t)n);
     for (int64_t k = 0; k < n; k++) {
         int d = cnt[key[k]]++;
-        sx[d] = x[k]; sy[d] = y[k]; svx[d] = vx[k]; svy[d] = vy[k];
+        buf[d] = make_float4(x[k], y[k], vx[k], vy[k]);
     }
-    int c = s.cur;
-    size_t b = (size_t)n * sizeof(float);
-    HIPTRY(&r->err, hipMemcpy(h.px[species][c], sx.data(), b, hipMemcpyHostToDevice));
-    HIPTRY(&r->err, hipMemcpy(h.py[species][c], sy.data(), b, hipMemcpyHostToDevice));
-    HIPTRY(&r->err, hipMemcpy(h.pvx[species][c], svx.data(), b, hipMemcpyHostToDevice));
-    HIPTRY(&r->err, hipMemcpy(h.pvy[species][c], svy.data(), b, hipMemcpyHostToDevice));
-    HIPTRY(&r->err, hipMemcpy(h.off[species][c], off.data(), off.size() * sizeof(int),
+    HIPTRY(&r->err, hipMemcpy(h.part[species][s.cur], buf.data(), (size_t)n * sizeof(float4),
+                              hipMemcpyHostToDevice));
+    HIPTRY(&r->err, hipMemcpy(h.off[species][s.cur], off.data(), off.size() * sizeof(int),
                               hipMemcpyHostToDevice));
     s.N[species] = (int)n;
     s.N_new[species] = (int)n;

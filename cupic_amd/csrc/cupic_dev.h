@@ -1,8 +1,9 @@
 // cupic_dev.h -- device-side data layout and shared device functions.
 //
 // Layout in HBM (per run):
-//   particles   FP32 SoA x, y, vx, vy per species, two buffers (ping-pong),
-//               sorted by cell key jc*ncx + ic with per-cell offsets.
+//   particles   FP32 (x, y, vx, vy) as one 16-byte record per particle (AoS),
+//               per species, two buffers (ping-pong), sorted by cell key
+//               jc*ncx + ic with per-cell offsets.
 //   grids       periodic in x, [ncy+1][ncx]: node (i, j) at j*ncx + i.
 //               charge: int64 fixed point (2^-32 e); rhs: FP32 and FP64;
 //               phi: FP32 ping-pong (modes 0, 1) / FP64 master + FP32
@@ -105,7 +106,7 @@ struct RunDev {
     SpecConst sp[2];
     unsigned long long seed, run_id;
     unsigned int quirks;
-    float *px[2][2], *py[2][2], *pvx[2][2], *pvy[2][2];   // [species][buffer]
+    float4 *part[2][2];             // [species][buffer], (x, y, vx, vy) per particle
     int *off[2][2];                 // [species][buffer], ncells + 1
     int *cnt[2];                    // [9][ncells] arrivals per (direction, old cell)
     int *base[2];                   // [10][ncells] slot bases per new cell
@@ -146,6 +147,19 @@ template <class T>
 __device__ __forceinline__ CUPIC_GLOBAL T *gp(T *p)
 {
     return (CUPIC_GLOBAL T *)p;
+}
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+// one particle record (x, y, vx, vy): a single 16-byte access per lane
+__device__ __forceinline__ v4f ld4(const float4 *p, int i)
+{
+    return ((const CUPIC_GLOBAL v4f *)p)[i];
+}
+
+__device__ __forceinline__ void st4(float4 *p, int i, v4f v)
+{
+    ((CUPIC_GLOBAL v4f *)p)[i] = v;
 }
 
 // float2 is a class type that cannot be copied through an address-space
@@ -259,10 +273,17 @@ __device__ __forceinline__ void cell_of(const G &R, float x, float y, float &fx,
 {
     float tx = x * R.inv_ds;
     float ty = y * R.inv_ds;
-    float bx = fminf(fmaxf(floorf(tx), 0.0f), (float)(R.ncx - 1));
-    float by = fminf(fmaxf(floorf(ty), 0.0f), (float)(R.ncy - 1));
-    fx = fminf(fmaxf(tx - bx, 0.0f), 1.0f);
-    fy = fminf(fmaxf(ty - by, 0.0f), 1.0f);
+    float bx = floorf(tx), by = floorf(ty);
+    fx = tx - bx;
+    fy = ty - by;
+    // inside the grid the clamps below change nothing; outside (or NaN) they
+    // pin the cell to the edge and the fractions into [0, 1]
+    if (__builtin_expect(!(tx >= 0.0f && tx < (float)R.ncx && ty >= 0.0f && ty < (float)R.ncy), 0)) {
+        bx = fminf(fmaxf(bx, 0.0f), (float)(R.ncx - 1));
+        by = fminf(fmaxf(by, 0.0f), (float)(R.ncy - 1));
+        fx = fminf(fmaxf(tx - bx, 0.0f), 1.0f);
+        fy = fminf(fmaxf(ty - by, 0.0f), 1.0f);
+    }
     ic = (int)bx;
     jc = (int)by;
 }

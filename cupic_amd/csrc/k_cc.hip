@@ -260,8 +260,8 @@ __global__ __launch_bounds__(kBlock) void k_scan_c(const RunDev *rds, const int 
 }
 
 // Far list, after the scan: each entry takes a place in its cell's far
-// region (any order) and writes its list index there, in the output buffer
-// that the far scatter overwrites last.
+// region (any order) and writes its list index into the first word of that
+// slot of the output buffer, which the far scatter overwrites last.
 __global__ __launch_bounds__(kBlock) void k_far_place(const RunDev *rds, const int *start,
                                                       int nruns)
 {
@@ -276,7 +276,7 @@ __global__ __launch_bounds__(kBlock) void k_far_place(const RunDev *rds, const i
     int slot = atomicAdd(&R.far_cur[s][key], 1);
     int pos = R.base[s][9 * R.ncells + key] + slot;
     if ((unsigned)pos < (unsigned)R.cap)
-        ((int *)R.py[s][st->cur ^ 1])[pos] = e;
+        ((int *)R.part[s][st->cur ^ 1])[4 * pos] = e;  // first word of the slot
 }
 
 // Stable rank of a far entry within its cell: entries of the same cell with
@@ -295,10 +295,10 @@ __global__ __launch_bounds__(kBlock) void k_far_rank(const RunDev *rds, const in
     int key = far[e].key, idx = far[e].idx;
     int b = R.base[s][9 * R.ncells + key];
     int n = R.off[s][st->cur ^ 1][key + 1] - b;
-    const int *slots = (const int *)R.py[s][st->cur ^ 1];
+    const int *slots = (const int *)R.part[s][st->cur ^ 1];
     int rank = 0;
     for (int u = 0; u < n; u++) {
-        int q = slots[b + u];
+        int q = slots[4 * (b + u)];
         if ((unsigned)q < (unsigned)F) rank += far[q].idx < idx;
     }
     R.far[s][e].rank = rank;
@@ -323,10 +323,7 @@ __global__ __launch_bounds__(kBlock) void k_far_scatter(const RunDev *rds, const
         atomicOr(&st->err, kErrSortInvariant);
         return;
     }
-    R.px[s][o][dest] = rec.x;
-    R.py[s][o][dest] = rec.y;
-    R.pvx[s][o][dest] = rec.vx;
-    R.pvy[s][o][dest] = rec.vy;
+    st4(R.part[s][o], dest, v4f{rec.x, rec.y, rec.vx, rec.vy});
     float fx, fy;
     int ic, jc;
     cell_of(R, rec.x, rec.y, fx, fy, ic, jc);

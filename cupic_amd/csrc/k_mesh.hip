@@ -168,6 +168,21 @@ __global__ __launch_bounds__(kBlock) void k_jacobi_sweep(const RunDev *rds, cons
     });
 }
 
+// Neighbour values one lane away, as VALU data-parallel-primitive moves:
+// wave_shr:1 gives lane l the value of lane l-1, wave_shl:1 that of lane
+// l+1 (lanes 0 and 63 keep their own value; they are never updated).
+__device__ __forceinline__ float lane_prev(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v),
+                                                      0x138, 0xF, 0xF, false));
+}
+
+__device__ __forceinline__ float lane_next(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v),
+                                                      0x130, 0xF, 0xF, false));
+}
+
 // nsw <= kJacH - 1 Jacobi sweeps s0 .. s0+nsw-1 in one launch (modes 1, 2).
 // A workgroup holds a 64 x (4 * kJacRows) node region -- a kJacTX x kJacTY
 // tile with a kJacH-node halo -- in registers: lane = region column, each
@@ -244,8 +259,8 @@ __global__ __launch_bounds__(kBlock) void k_jacobi_block(const RunDev *rds, cons
 #pragma unroll
         for (int i = 0; i < NR; i++) {
             const float cur = v[i];
-            float W = __shfl_up(cur, 1, kWave);
-            float E = __shfl_down(cur, 1, kWave);
+            float W = lane_prev(cur);
+            float E = lane_next(cur);
             float N = i < NR - 1 ? v[i + 1] : above;
             float nvv = 0.25f * ((((b[i] + W) + E) + N) + prev);
             prev = cur;
@@ -271,8 +286,8 @@ __global__ __launch_bounds__(kBlock) void k_jacobi_block(const RunDev *rds, cons
     double rmax = 0.0;
 #pragma unroll
     for (int i = 0; i < NR; i++) {
-        float W = __shfl_up(v[i], 1, kWave);
-        float E = __shfl_down(v[i], 1, kWave);
+        float W = lane_prev(v[i]);
+        float E = lane_next(v[i]);
         float S = i > 0 ? v[i - 1] : below;
         float N = i < NR - 1 ? v[i + 1] : above;
         int lr = NR * w + i, gy = row0 + i;

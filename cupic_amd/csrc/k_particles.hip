@@ -64,10 +64,7 @@ __global__ __launch_bounds__(kBlock) void k_create(const RunDev *rds, const int 
     float yf = (float)y;
     for (int q = 0; q < 64 && floorf(yf * R.inv_ds) > (float)row; q++) yf = nextafterf(yf, -INFINITY);
     for (int q = 0; q < 64 && floorf(yf * R.inv_ds) < (float)row; q++) yf = nextafterf(yf, INFINITY);
-    R.px[s][1][g] = xf;
-    R.py[s][1][g] = yf;
-    R.pvx[s][1][g] = (float)vx;
-    R.pvy[s][1][g] = (float)vy;
+    st4(R.part[s][1], g, v4f{xf, yf, (float)vx, (float)vy});
 }
 
 constexpr int kInitMaxNcx = 4096;
@@ -88,14 +85,15 @@ __global__ __launch_bounds__(kBlock) void k_init_sort(const RunDev *rds, const i
     int ncx = R.ncx;
     int pr = R.per_row;
     int first = row * pr;
-    const float *X = R.px[s][1], *Y = R.py[s][1], *VX = R.pvx[s][1], *VY = R.pvy[s][1];
-    float *OX = R.px[s][0], *OY = R.py[s][0], *OVX = R.pvx[s][0], *OVY = R.pvy[s][0];
+    const float4 *IN = R.part[s][1];
+    float4 *OUT = R.part[s][0];
     for (int c = threadIdx.x; c < ncx; c += kBlock) { hist[c] = 0; runc[c] = 0; }
     __syncthreads();
     for (int q = threadIdx.x; q < pr; q += kBlock) {
         float fx, fy;
         int ic, jc;
-        cell_of(R, X[first + q], Y[first + q], fx, fy, ic, jc);
+        v4f p = ld4(IN, first + q);
+        cell_of(R, p.x, p.y, fx, fy, ic, jc);
         atomicAdd(&hist[ic], 1);
     }
     __syncthreads();
@@ -123,7 +121,11 @@ __global__ __launch_bounds__(kBlock) void k_init_sort(const RunDev *rds, const i
         int ic = -1;
         float fx, fy;
         int jc;
-        if (valid) cell_of(R, X[first + q], Y[first + q], fx, fy, ic, jc);
+        v4f p = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+        if (valid) {
+            p = ld4(IN, first + q);
+            cell_of(R, p.x, p.y, fx, fy, ic, jc);
+        }
         chunk[threadIdx.x] = ic;
         __syncthreads();
         int rank = 0, after = 0;
@@ -131,10 +133,7 @@ __global__ __launch_bounds__(kBlock) void k_init_sort(const RunDev *rds, const i
             for (int t = 0; t < (int)threadIdx.x; t++) rank += (chunk[t] == ic);
             for (int t = threadIdx.x + 1; t < kBlock; t++) after += (chunk[t] == ic);
             int dest = first + hist[ic] + runc[ic] + rank;
-            OX[dest] = X[first + q];
-            OY[dest] = Y[first + q];
-            OVX[dest] = VX[first + q];
-            OVY[dest] = VY[first + q];
+            st4(OUT, dest, p);
         }
         __syncthreads();
         if (valid && after == 0) runc[ic] += rank + 1;
@@ -146,8 +145,8 @@ __global__ __launch_bounds__(kBlock) void k_init_sort(const RunDev *rds, const i
 
 // Adds the fixed-point CIC charge of the wave's depositing lanes to a
 // wave-private LDS node tile.  Lanes are grouped by tile cell; each group
-// reduces (sum wx, sum wy, count) and sum wx*wy, from which the four corner
-// sums follow exactly:
+// reduces (sum wx | sum wy << 32) and (sum wx*wy | count << 48), from which
+// the four corner sums follow exactly:
 //   c00 = 2^32 n - 2^16 (Swx + Swy) + Swxwy, c10 = 2^16 Swx - Swxwy,
 //   c01 = 2^16 Swy - Swxwy,                 c11 = Swxwy.
 __device__ __forceinline__ void wave_deposit(unsigned long long *tile, int TW, int dkey,
@@ -155,8 +154,8 @@ __device__ __forceinline__ void wave_deposit(unsigned long long *tile, int TW, i
 {
     unsigned long long v1 = 0, v2 = 0;
     if (dkey >= 0) {
-        v1 = (unsigned long long)wx | ((unsigned long long)wy << 23) | (1ull << 46);
-        v2 = (unsigned long long)wx * (unsigned long long)wy;
+        v1 = (unsigned long long)wx | ((unsigned long long)wy << 32);
+        v2 = (unsigned long long)wx * (unsigned long long)wy + (1ull << 48);
     }
     unsigned long long pending = __ballot(dkey >= 0);
     while (pending) {
@@ -167,10 +166,10 @@ __device__ __forceinline__ void wave_deposit(unsigned long long *tile, int TW, i
         unsigned long long a1 = wave_sum_u64(mine ? v1 : 0ull);
         unsigned long long a2 = wave_sum_u64(mine ? v2 : 0ull);
         if (lane == leader) {
-            long long sx = (long long)(a1 & ((1ull << 23) - 1));
-            long long sy = (long long)((a1 >> 23) & ((1ull << 23) - 1));
-            long long n = (long long)(a1 >> 46);
-            long long sxy = (long long)a2;
+            long long sx = (long long)(a1 & 0xFFFFFFFFull);
+            long long sy = (long long)(a1 >> 32);
+            long long n = (long long)(a2 >> 48);
+            long long sxy = (long long)(a2 & ((1ull << 48) - 1));
             long long c11 = sxy;
             long long c10 = (sx << 16) - sxy;
             long long c01 = (sy << 16) - sxy;
@@ -237,7 +236,7 @@ __global__ __launch_bounds__(kBlock) void k_deposit(const RunDev *rds, const int
     const int *off = R.off[T.s][cur];
     int c0 = T.j0 * R.ncx + T.col0;
     int p0 = off[c0], p1 = off[c0 + T.nc];
-    const float *X = R.px[T.s][cur], *Y = R.py[T.s][cur];
+    const float4 *P = R.part[T.s][cur];
     long long sgn = T.s == 0 ? -1 : 1;
     for (int i0 = p0; i0 < p1; i0 += kWave) {
         int i = i0 + lane;
@@ -246,7 +245,8 @@ __global__ __launch_bounds__(kBlock) void k_deposit(const RunDev *rds, const int
         if (i < p1) {
             float fx, fy;
             int ic, jc;
-            cell_of(R, X[i], Y[i], fx, fy, ic, jc);
+            v4f p = ld4(P, i);
+            cell_of(R, p.x, p.y, fx, fy, ic, jc);
             int lc = ic - T.col0 + 1, lr = jc - (T.j0 - 1);
             fixed_weights(fx, fy, wx, wy);
             if (lr >= 0 && lr <= 2 && lc >= 0 && lc <= T.nc + 1) {
@@ -278,47 +278,95 @@ __global__ __launch_bounds__(kBlock) void k_half_kick(const RunDev *rds, const i
     int cur = st->cur;
     int n = st->N[s];
     float hk = R.sp[s].hkick;
-    const float *X = R.px[s][cur], *Y = R.py[s][cur];
-    float *VX = R.pvx[s][cur], *VY = R.pvy[s][cur];
+    float4 *P = R.part[s][cur];
     for (int i = (local - s * bps) * kBlock + threadIdx.x; i < n; i += bps * kBlock) {
         float fx, fy, ex, ey;
         int ic, jc;
-        cell_of(R, X[i], Y[i], fx, fy, ic, jc);
+        v4f p = ld4(P, i);
+        cell_of(R, p.x, p.y, fx, fy, ic, jc);
         gather(R, R.E, fx, fy, ic, jc, ex, ey);
-        VX[i] = VX[i] + hk * ex;
-        VY[i] = VY[i] + hk * ey;
+        p.z = p.z + hk * ex;
+        p.w = p.w + hk * ey;
+        st4(P, i, p);
     }
 }
 
 // ---------------------------------------------------------------- push passes
 
+// One DPP move of a 64-bit value (two 32-bit halves); lanes whose source
+// is out of range or whose row is masked off read 0.
+template <int kCtrl, int kRowMask>
+__device__ __forceinline__ unsigned long long dpp_u64(unsigned long long v)
+{
+    int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)v, kCtrl, kRowMask, 0xF, true);
+    int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(v >> 32), kCtrl, kRowMask, 0xF, true);
+    return ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo;
+}
+
 // Segmented inclusive prefix sum over the wave (segments are runs of lanes
-// that start at seg_start); lanes outside [seg_start, lane] do not add.
+// that start at seg_start): Hillis-Steele within rows of 16 lanes with
+// row_shr, then across rows with row_bcast:15 and row_bcast:31; a lane adds
+// a partial sum only when its source lane lies in its own segment.
 __device__ __forceinline__ unsigned long long seg_scan_u64(unsigned long long v, int lane,
                                                            int seg_start)
 {
-#pragma unroll
-    for (int o = 1; o < kWave; o <<= 1) {
-        unsigned long long u = __shfl_up(v, o, kWave);
-        if (lane - o >= seg_start) v += u;
-    }
+    const int r = lane & 15;
+    unsigned long long u;
+    u = dpp_u64<0x111, 0xF>(v);  // row_shr:1
+    if (r >= 1 && lane - 1 >= seg_start) v += u;
+    u = dpp_u64<0x112, 0xF>(v);  // row_shr:2
+    if (r >= 2 && lane - 2 >= seg_start) v += u;
+    u = dpp_u64<0x114, 0xF>(v);  // row_shr:4
+    if (r >= 4 && lane - 4 >= seg_start) v += u;
+    u = dpp_u64<0x118, 0xF>(v);  // row_shr:8
+    if (r >= 8 && lane - 8 >= seg_start) v += u;
+    u = dpp_u64<0x142, 0xA>(v);  // row_bcast:15 -> rows 1 and 3
+    if ((lane >> 4) & 1 && (lane & ~15) - 1 >= seg_start) v += u;
+    u = dpp_u64<0x143, 0xC>(v);  // row_bcast:31 -> rows 2 and 3
+    if (lane >= 32 && 31 >= seg_start) v += u;
     return v;
+}
+
+// popcount of m over the lanes below this one
+__device__ __forceinline__ int mbcnt64(unsigned long long m)
+{
+    return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                          __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+}
+
+// value of lane l-1 (lane 0 keeps its own), a VALU wave_shr:1 move
+__device__ __forceinline__ int dpp_prev_int(int v)
+{
+    return __builtin_amdgcn_update_dpp(v, v, 0x138, 0xF, 0xF, false);
 }
 
 // Adds one particle group's fixed-point charge (reduced sums, see
 // wave_deposit) to the four corners of tile cell key.
+// Packing: a1 = sum wx | sum wy << 32, a2 = sum wx*wy | count << 48; exact
+// while a group holds fewer than 65536 particles.
 __device__ __forceinline__ void tile_add(unsigned long long *tile, int TW, int key,
                                          unsigned long long a1, unsigned long long a2,
                                          long long sgn)
 {
-    long long sx = (long long)(a1 & ((1ull << 23) - 1));
-    long long sy = (long long)((a1 >> 23) & ((1ull << 23) - 1));
-    long long n = (long long)(a1 >> 46);
-    long long sxy = (long long)a2;
+    long long sx = (long long)(a1 & 0xFFFFFFFFull);
+    long long sy = (long long)(a1 >> 32);
+    long long n = (long long)(a2 >> 48);
+    long long sxy = (long long)(a2 & ((1ull << 48) - 1));
     atomicAdd(&tile[key], (unsigned long long)(sgn * ((n << 32) - ((sx + sy) << 16) + sxy)));
     atomicAdd(&tile[key + 1], (unsigned long long)(sgn * ((sx << 16) - sxy)));
     atomicAdd(&tile[key + TW], (unsigned long long)(sgn * ((sy << 16) - sxy)));
     atomicAdd(&tile[key + TW + 1], (unsigned long long)(sgn * sxy));
+}
+
+// Reduces the per-lane stayer sums of old cell cl over the wave and adds
+// them to the tile cell of cl (stayers keep their cell).
+__device__ __forceinline__ void flush_stayers(unsigned long long *tile, int TW, int cl,
+                                              unsigned long long a1, unsigned long long a2,
+                                              long long sgn, int lane)
+{
+    a1 = seg_scan_u64(a1, lane, 0);
+    a2 = seg_scan_u64(a2, lane, 0);
+    if (lane == kWave - 1 && (a2 >> 48) != 0ull) tile_add(tile, TW, 1 * TW + (cl + 1), a1, a2, sgn);
 }
 
 // LDS tile of E on the two node rows of a strip: columns col0 .. col0+nc.
@@ -406,10 +454,8 @@ __device__ __forceinline__ void push_task(const RunDev &R, RunState *st, const T
     const CUPIC_GLOBAL int *off = gp(R.off[s][cur]);
     const int c0 = j0 * R.ncx + col0;
     const int p0 = off[c0], p1 = off[c0 + nc];
-    const CUPIC_GLOBAL float *X = gp(R.px[s][cur]), *Y = gp(R.py[s][cur]);
-    const CUPIC_GLOBAL float *VX = gp(R.pvx[s][cur]), *VY = gp(R.pvy[s][cur]);
-    CUPIC_GLOBAL float *OX = gp(R.px[s][cur ^ 1]), *OY = gp(R.py[s][cur ^ 1]);
-    CUPIC_GLOBAL float *OVX = gp(R.pvx[s][cur ^ 1]), *OVY = gp(R.pvy[s][cur ^ 1]);
+    const float4 *PIN = R.part[s][cur];
+    float4 *POUT = R.part[s][cur ^ 1];
     const int ncells = R.ncells;
     const float2 *E = R.E;
     const float qdtm = R.sp[s].qdtm;
@@ -441,17 +487,17 @@ __device__ __forceinline__ void push_task(const RunDev &R, RunState *st, const T
     const unsigned long long upto = lt | (1ull << lane);
     unsigned long long probe_n = 0, top_n = 0;
     bool bad = false;
-    float nx_ = 0.0f, ny_ = 0.0f, nvx_ = 0.0f, nvy_ = 0.0f;
-    if (p0 + lane < p1) {
-        nx_ = X[p0 + lane]; ny_ = Y[p0 + lane]; nvx_ = VX[p0 + lane]; nvy_ = VY[p0 + lane];
-    }
+    unsigned long long acc1 = 0ull, acc2 = 0ull;  // stayer sums of old cell acc_cl
+    int acc_cl = -1, acc_n = 0;
+    // the next chunk is in flight while one is processed
+    v4f nxt = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+    if (p0 + lane < p1) nxt = ld4(PIN, p0 + lane);
     for (int i0 = p0; i0 < p1; i0 += kWave) {
         const int i = i0 + lane;
         const bool valid = i < p1;
-        const float x = nx_, y = ny_, vx = nvx_, vy = nvy_;
-        if (i + kWave < p1) {  // prefetch the next chunk
-            nx_ = X[i + kWave]; ny_ = Y[i + kWave]; nvx_ = VX[i + kWave]; nvy_ = VY[i + kWave];
-        }
+        const v4f cur4 = nxt;
+        const float x = cur4.x, y = cur4.y, vx = cur4.z, vy = cur4.w;
+        if (i + kWave < p1) nxt = ld4(PIN, i + kWave);
         bool in_strip;
         Pushed o = push_tile(G, Et, nc, col0, j0, E, qdtm, x, y, vx, vy, in_strip);
         int dir = valid ? o.dir : (int)kDirNone;
@@ -461,28 +507,48 @@ __device__ __forceinline__ void push_task(const RunDev &R, RunState *st, const T
             bad = true;
         }
         // rank within (old cell, direction); lanes of one old cell are contiguous
-        unsigned long long mymask = 0;
-#pragma unroll
-        for (int dd = 0; dd < 9; dd++) {
-            unsigned long long m = __ballot(dir == dd);
-            if (dir == dd) mymask = m;
-        }
-        int clp = __shfl_up(cl, 1, kWave);
+        int clp = dpp_prev_int(cl);
         bool head = valid && (lane == 0 || cl != clp);
         unsigned long long heads = __ballot(head);
-        int seg_start = valid ? 63 - __clzll((long long)(heads & upto)) : lane;
+        unsigned long long near_m = __ballot(dir < 9);
         int rank = 0;
-        bool last = false;
-        if (dir < 9) {
-            unsigned long long after = heads & ~upto;
-            unsigned long long hi = after ? ((1ull << (__ffsll((long long)after) - 1)) - 1ull) : ~0ull;
-            unsigned long long seg = hi & ~((1ull << seg_start) - 1ull);
-            unsigned long long grp = mymask & seg;
-            rank = cntw[cl * 9 + dir] + __popcll(grp & lt);
-            last = (grp & ~upto) == 0ull;
+        int seg_start = lane;
+        const bool one_cell = (heads == 1ull);
+        int cl0 = 0;
+        if (one_cell) {
+            // the whole chunk lies in one old cell: per direction present,
+            // rank = counter + earlier lanes with that direction
+            seg_start = 0;
+            cl0 = __builtin_amdgcn_readfirstlane(cl);
+            unsigned long long pend = near_m;
+            while (pend) {
+                int d = __builtin_amdgcn_readlane(dir, __ffsll((long long)pend) - 1);
+                unsigned long long m = __ballot(dir == d);
+                int base_c = cntw[cl0 * 9 + d];
+                if (dir == d) rank = base_c + mbcnt64(m);
+                if (lane == 63 - __clzll((long long)m)) cntw[cl0 * 9 + d] = base_c + __popcll(m);
+                pend &= ~m;
+            }
+        } else {
+            unsigned long long mymask = 0;
+#pragma unroll
+            for (int dd = 0; dd < 9; dd++) {
+                unsigned long long m = __ballot(dir == dd);
+                if (dir == dd) mymask = m;
+            }
+            seg_start = valid ? 63 - __clzll((long long)(heads & upto)) : lane;
+            bool last = false;
+            if (dir < 9) {
+                unsigned long long after = heads & ~upto;
+                unsigned long long hi = after ? ((1ull << (__ffsll((long long)after) - 1)) - 1ull) : ~0ull;
+                unsigned long long seg = hi & ~((1ull << seg_start) - 1ull);
+                unsigned long long grp = mymask & seg;
+                rank = cntw[cl * 9 + dir] + __popcll(grp & lt);
+                last = (grp & ~upto) == 0ull;
+            }
+            wave_lds_sync();
+            if (dir < 9 && last) cntw[cl * 9 + dir] = rank + 1;
         }
-        wave_lds_sync();
-        if (dir < 9 && last) cntw[cl * 9 + dir] = rank + 1;
         wave_lds_sync();
         if (!kScatter) {
             unsigned long long fm = __ballot(dir == kDirFar);
@@ -514,35 +580,54 @@ __device__ __forceinline__ void push_task(const RunDev &R, RunState *st, const T
             if (dir < 9) {
                 int dest = rank;  // counters hold slot base + rank
                 if ((unsigned)dest < (unsigned)cap) {
-                    OX[dest] = o.x;
-                    OY[dest] = o.y;
-                    OVX[dest] = o.vx;
-                    OVY[dest] = o.vy;
+                    st4(POUT, dest, v4f{o.x, o.y, o.vx, o.vy});
                 } else {
                     bad = true;
                 }
                 fixed_weights(o.fx, o.fy, wx, wy);
             }
-            // stayers: one segmented sum per old cell; movers: their own atomics
             const bool stay = (dir == 4);
             unsigned long long v1 = 0, v2 = 0;
             if (dir < 9) {
-                v1 = (unsigned long long)wx | ((unsigned long long)wy << 23) | (1ull << 46);
-                v2 = (unsigned long long)wx * (unsigned long long)wy;
+                v1 = (unsigned long long)wx | ((unsigned long long)wy << 32);
+                v2 = (unsigned long long)wx * (unsigned long long)wy + (1ull << 48);
             }
-            unsigned long long s1 = seg_scan_u64(stay ? v1 : 0ull, lane, seg_start);
-            unsigned long long s2 = seg_scan_u64(stay ? v2 : 0ull, lane, seg_start);
-            unsigned long long nxt_head = __ballot(head) & ~upto;
-            bool seg_last = valid && (nxt_head ? (__ffsll((long long)nxt_head) - 1 == lane + 1)
-                                               : (i + 1 >= p1 || lane == kWave - 1));
-            if (seg_last && (s1 >> 46) != 0ull && in_strip)
-                tile_add(tile, TW, 1 * TW + (cl + 1), s1, s2, sgn);
+            // stayers: per-lane sums while the wave stays in one old cell,
+            // reduced once per cell (or every 60000 particles); chunks that
+            // span cells use one segmented sum per cell; movers add their own
+            const int nstay = __popcll(__ballot(stay));
+            if (one_cell && cl0 == acc_cl && acc_n + nstay < 60000) {
+                if (stay) { acc1 += v1; acc2 += v2; }
+                acc_n += nstay;
+            } else {
+                if (acc_cl >= 0) {
+                    flush_stayers(tile, TW, acc_cl, acc1, acc2, sgn, lane);
+                    acc1 = 0ull;
+                    acc2 = 0ull;
+                    acc_cl = -1;
+                    acc_n = 0;
+                }
+                if (one_cell) {
+                    acc_cl = cl0;
+                    acc_n = nstay;
+                    if (stay) { acc1 = v1; acc2 = v2; }
+                } else {
+                    unsigned long long s1 = seg_scan_u64(stay ? v1 : 0ull, lane, seg_start);
+                    unsigned long long s2 = seg_scan_u64(stay ? v2 : 0ull, lane, seg_start);
+                    unsigned long long nxt_head = heads & ~upto;
+                    bool seg_last = valid && (nxt_head ? (__ffsll((long long)nxt_head) - 1 == lane + 1)
+                                                       : (i + 1 >= p1 || lane == kWave - 1));
+                    if (seg_last && (s2 >> 48) != 0ull && in_strip)
+                        tile_add(tile, TW, 1 * TW + (cl + 1), s1, s2, sgn);
+                }
+            }
             if (dir < 9 && !stay) {
                 int key = (o.njc - (j0 - 1)) * TW + (cl + o.di + 1);
                 tile_add(tile, TW, key, v1, v2, sgn);
             }
         }
     }
+    if (kScatter && acc_cl >= 0) flush_stayers(tile, TW, acc_cl, acc1, acc2, sgn, lane);
     wave_lds_sync();
     if (!kScatter) {
         CUPIC_GLOBAL int *cnt = gp(R.cnt[s]);
