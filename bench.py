@@ -1,6 +1,6 @@
 """Full PIC cycle throughput on one MI355X.
 
-    python bench.py [--steps K] [--warmup W] [--config prod|small] [--cpu-steps C]
+    python bench.py [--steps K] [--warmup W] [--config prod|small] [--cpu-seconds S]
 
 Workload (default, "prod"): 1024x512 cells, ds 0.5, dt 0.05, 200 particles
 per cell per species (2 x 104,857,600), mi/me 1836, Te/Ti 10, probe bias
@@ -38,11 +38,10 @@ def workload_name(cfg, p):
             f"Jacobi FP64 residual {p.jacobi_tol:g} (max {p.jacobi_max_sweeps} sweeps)")
 
 
-def cpu_baseline(cfg, steps):
-    """The oracle's OpenMP FP64 cycle on the same workload, timed for a few
-    steps after its own init (the CPU checker, never the measured path)."""
-    import ctypes as C
-
+def cpu_baseline(cfg, min_seconds, max_steps):
+    """The oracle's OpenMP FP64 cycle on the same workload (the CPU checker,
+    never the measured path): full cycles after its own init, repeated until
+    min_seconds of stepping have passed (at least 2, at most max_steps)."""
     from oracle import oracle as O
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
     os.environ.setdefault("OMP_NUM_THREADS", str(threads))
@@ -52,16 +51,34 @@ def cpu_baseline(cfg, steps):
     sim.init()
     t = p.dt
     n = 0
+    steps = 0
     t0 = time.perf_counter()
-    for _ in range(steps):
+    while steps < max_steps and (steps < 2 or time.perf_counter() - t0 < min_seconds):
         n += sim.count(0) + sim.count(1)
         sim.step(t)
         t += p.dt
+        steps += 1
     el = time.perf_counter() - t0
     del sim
     return {"value": n / el, "unit": "particle-steps/s", "cores": threads, "kind": "port",
             "sample": f"{steps} full cycles of the same workload after init "
-                      f"(FP64, OpenMP, {threads} threads), {el:.1f} s"}
+                      f"(FP64, OpenMP, {threads} threads, Jacobi capped at 500 sweeps), "
+                      f"{el:.1f} s"}
+
+
+def profiled_traffic(kernel):
+    """HBM bytes per particle of a kernel, from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (copy-calibrated), or None."""
+    path = os.path.join(ROOT, "profiles", "scatter_push_1024x512_200ppc_hbm_traffic.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                w = line.split()
+                if len(w) == 3 and w[0] == "traffic_bytes_per_particle" and w[1] == kernel:
+                    return float(w[2]), os.path.relpath(path, ROOT)
+    except OSError:
+        pass
+    return None, None
 
 
 def main():
@@ -70,7 +87,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="prod", choices=sorted(CONFIGS))
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--cpu-max-steps", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-timing", action="store_true")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -173,21 +191,29 @@ def main():
         bytes_per_particle = 32.0 if dom == "scatter" else 16.0
         dur = per[dom] * 1e-3
         ach = bytes_per_particle * npart / dur if dur > 0 else 0.0
+        tpp, src = profiled_traffic("k_" + dom) if a.config == "prod" else (None, None)
         out["roofline"] = {"bound": "hbm", "kernel": "k_" + dom, "achieved": ach / 1e9,
                            "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": ach / HBM_PEAK,
-                           "traffic": None,
+                           "traffic": tpp * npart if tpp else None,
+                           "traffic_unit": "HBM bytes per launch",
+                           "traffic_source": src,
                            "bytes_per_particle": bytes_per_particle,
                            "avg_launch_ms": per[dom]}
     out["cycle_roofline"] = {"bytes_per_step": cycle_bytes,
                              "achieved_GBps": cycle_bytes / (wall / a.steps) / 1e9,
                              "frac": cycle_bytes / (wall / a.steps) / HBM_PEAK,
                              "formula": "84*(Ne+Ni) + nodes*(32 + 12*S)"}
+    design_bytes = 48.0 * npart + nodes * (32.0 + 12.0 * S)
+    out["cycle_roofline_design"] = {"bytes_per_step": design_bytes,
+                                    "frac": design_bytes / (wall / a.steps) / HBM_PEAK,
+                                    "formula": "48*(Ne+Ni) + nodes*(32 + 12*S): push_rank reads "
+                                               "16 B, scatter reads 16 B and writes 16 B"}
     out["probe"] = {"total_absorbed": [int(v) for v in tally["total_probe"]],
                     "charge_collected": tally["charge"]}
     run.close()
-    if not a.no_cpu and a.cpu_steps > 0:
+    if not a.no_cpu and a.cpu_max_steps > 0:
         print("# cpu baseline ...", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(cfg, a.cpu_steps)
+        out["cpu_baseline"] = cpu_baseline(cfg, a.cpu_seconds, a.cpu_max_steps)
     print(json.dumps(out), flush=True)
 
 
