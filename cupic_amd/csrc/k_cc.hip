@@ -65,7 +65,8 @@ __global__ __launch_bounds__(kBlock) void k_clock(const RunDev *rds, const int *
 // y = Ly, vx ~ N(0, sigma), vy = -Rayleigh(sigma); E interpolated from the
 // top node row (disty = 1); partial push x += (t+dt-tin) v and
 // v += q (t+dt/2-tin) E/m (Q7; the committed sign for both species is kept
-// behind a quirk flag); wrapped in x, rounded to FP32, wrapped again.
+// behind a quirk flag); wrapped in x, rounded to FP32, wrapped again; a y
+// rounded up to Ly becomes the largest float below Ly.
 __global__ __launch_bounds__(kBlock) void k_inject(const RunDev *rds, const int *start,
                                                    int nruns)
 {
@@ -118,6 +119,8 @@ __global__ __launch_bounds__(kBlock) void k_inject(const RunDev *rds, const int 
     FarRec rec;
     rec.x = xf;
     rec.y = (float)y;
+    // y < Ly in FP64 (vy < 0), but rounding can reach Ly: keep it inside
+    if (rec.y >= R.Ly_f) rec.y = __int_as_float(__float_as_int(R.Ly_f) - 1);
     rec.vx = (float)vx;
     rec.vy = (float)vy;
     float fx, fy;

@@ -946,7 +946,8 @@ int64_t orc_mirror_step(const orc_f32c *c, int species, int64_t n, const float *
 }
 
 /* Injection evaluated in FP64 from an FP32 field, then rounded; x is wrapped
- * once more in FP32 in case rounding reached Lx. */
+ * once more in FP32 in case rounding reached Lx, and a y that rounded up to
+ * Ly (it is below Ly in FP64: vy < 0) becomes the largest float below Ly. */
 void orc_inject_f32(const orc_params *p, int species, uint64_t first_index, int n_in,
                     double t, const double *tin_seq, const float *Exy, float *x, float *y,
                     float *vx, float *vy)
@@ -964,7 +965,9 @@ void orc_inject_f32(const orc_params *p, int species, uint64_t first_index, int 
         float xf = (float)dx;
         if (xf < 0.0f) xf += c.Lx;
         if (xf >= c.Lx) xf -= c.Lx;
-        x[k] = xf; y[k] = (float)dy; vx[k] = (float)dvx; vy[k] = (float)dvy;
+        float yf = (float)dy;
+        if (yf >= c.Ly) yf = nextafterf(c.Ly, 0.0f);
+        x[k] = xf; y[k] = yf; vx[k] = (float)dvx; vy[k] = (float)dvy;
     }
     free(Ex); free(Ey);
 }

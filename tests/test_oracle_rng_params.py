@@ -121,3 +121,29 @@ def test_injected_velocity_distributions():
     fpt = 0.05 + p.dt
     assert np.allclose(y, p.Ly + (fpt - tin) * vy, rtol=0, atol=1e-12)
     assert np.all((x >= 0) & (x < p.Lx))
+
+
+def test_injected_fp32_positions_stay_inside():
+    """Rounded to FP32, an injected y just below Ly can reach Ly (256 here, whose
+    ulp is 2^-15): it must become the largest float below Ly, never Ly."""
+    p = O.make_params(ncx=64, ncy=512)
+    n = 200000
+    t = 0.05
+    tin = np.linspace(t + p.dt, t, n, endpoint=False)   # partial push down to 0
+    E = np.zeros(2 * O.nodes(p), np.float32)
+    out = [np.empty(n, np.float32) for _ in range(4)]
+    L.orc_inject_f32(p, 0, 0, n, t, tin, E, *out)
+    x, y, vx, vy = out
+    Ly = np.float32(p.Ly)
+    below = np.nextafter(Ly, np.float32(0))
+    assert np.all((x >= 0) & (x < np.float32(p.Lx)))
+    assert np.all((y >= 0) & (y < Ly))
+    assert np.count_nonzero(y == below) > 0       # the clamp was exercised
+    # the FP64 positions round to the same floats wherever they stay below Ly
+    y64 = np.empty(n)
+    o64 = [np.empty(n) for _ in range(4)]
+    nodes = O.nodes(p)
+    L.orc_inject_f64(p, 0, 0, n, t, tin, np.zeros(nodes), np.zeros(nodes), *o64)
+    y64 = o64[1]
+    keep = y64.astype(np.float32) < Ly
+    assert np.array_equal(y[keep], y64[keep].astype(np.float32))
