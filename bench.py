@@ -80,6 +80,177 @@ def profiled_traffic(kernel):
         pass
     return None, None
 
+def draw_batched(n_runs, seed):
+    """256 independent runs: ncx ~ U{64..512}, ncy ~ U{32..256}, ppc ~ U{20..100},
+    bias ~ U[-10, 2] kTe/e; other parameters as the small configuration."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n_runs):
+        out.append(dict(ncx=int(rng.integers(64, 513)), ncy=int(rng.integers(32, 257)),
+                        ppc=int(rng.integers(20, 101)), phi_p=float(rng.uniform(-10.0, 2.0)),
+                        mi=100.0, kti=1.0, run_id=k))
+    return out
+
+
+def batched_leg(cp, n_runs, steps, seq_steps, seed):
+    """One HIP-graph step advances all runs; baseline: the same runs stepped
+    one after another through the plain path (cupic_step per run)."""
+    draws = draw_batched(n_runs, seed)
+    runs = []
+    for d in draws:
+        p = cp.params(**d, jacobi_mode=cp.JACOBI_RESIDUAL, jacobi_tol=1e-6,
+                      jacobi_tol_init=1e-6, jacobi_max_sweeps=500, jacobi_check_every=50)
+        r = cp.Run(p)
+        r.init_sim()
+        runs.append(r)
+    for r in runs:
+        r.synchronize()
+    npart = sum(r.number_of_particles(0) + r.number_of_particles(1) for r in runs)
+    print(f"# batched: {n_runs} runs, {npart} particles", file=sys.stderr, flush=True)
+    b = cp.Batch(runs, graph=True)
+    b.step(2)
+    t0 = time.perf_counter()
+    b.step(steps)
+    wall_b = time.perf_counter() - t0
+    b.close()
+    # the runs continue on the plain path, one after another, from the time
+    # the batch reached (t accumulated as t += dt, like the device clock)
+    t_next = []
+    for r in runs:
+        t = 0.0
+        for _ in range(3 + steps):
+            t += r.p.dt
+        r.step(t)                     # warm-up of the plain path
+        r.synchronize()
+        t_next.append(t + r.p.dt)
+    t0 = time.perf_counter()
+    for r, t in zip(runs, t_next):
+        for _ in range(seq_steps):
+            r.step(t)
+            t += r.p.dt
+        r.synchronize()
+    wall_s = time.perf_counter() - t0
+    for r in runs:
+        r.close()
+    rs_b = n_runs * steps / wall_b
+    rs_s = n_runs * seq_steps / wall_s
+    return {"runs": n_runs, "particles": int(npart), "steps": steps,
+            "ms_per_batched_step": 1e3 * wall_b / steps,
+            "run_steps_per_s": rs_b, "particle_steps_per_s": npart * steps / wall_b,
+            "sequential_plain_run_steps_per_s": rs_s,
+            "sequential_plain_steps_per_run": seq_steps,
+            "speedup_vs_sequential_plain": rs_b / rs_s,
+            "workload": "256 runs, ncx~U{64..512}, ncy~U{32..256}, ppc~U{20..100}, "
+                        "phi_p~U[-10,2], mi/me 100, Te/Ti 1, Jacobi FP64 residual 1e-6 "
+                        "(max 500 sweeps), one HIP graph per batched step"}
+
+
+def draw_requests(n, rate, seed):
+    """I-V points: Poisson arrivals at `rate` per second, grid ncx ~ U{128..1024},
+    ncy ~ U{64..512}, ppc ~ U{20..200}, bias ~ U[-10, 2] kTe/e."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for k in range(n):
+        t += float(rng.exponential(1.0 / rate))
+        out.append((t, dict(ncx=int(rng.integers(128, 1025)), ncy=int(rng.integers(64, 513)),
+                            ppc=int(rng.integers(20, 201)),
+                            phi_p=float(rng.uniform(-10.0, 2.0)), run_id=1000 + k)))
+    return out
+
+
+def converged_rule(prev_net, ie, ii, rel):
+    gross = ii + ie
+    return gross > 0.0 and abs((ii - ie) - prev_net) / gross < rel
+
+
+def serving_leg(cp, n_req, rate, max_steps, seed):
+    """Continuous batching (cupic_server) against the same points served one
+    at a time through the plain path with the same convergence rule."""
+    import numpy as np
+    reqs = draw_requests(n_req, rate, seed)
+    window, rnd, rel = 500, 50, 0.005
+    srv = cp.Server(cp.server_config(max_steps=max_steps, window_steps=window,
+                                     round_steps=rnd, rel_change=rel))
+    start = srv.stats()["clock"]
+    pending = list(range(n_req))
+    ids = {}
+    arrival = {}
+    while True:
+        now = srv.stats()["clock"] - start
+        while pending and reqs[pending[0]][0] <= now:
+            k = pending.pop(0)
+            ids[k] = srv.submit(cp.params(**reqs[k][1]))
+            arrival[k] = reqs[k][0]
+        active, queued = srv.poll()
+        if not pending and active == 0 and queued == 0:
+            break
+        if active == 0 and queued == 0 and pending:
+            time.sleep(max(0.0, reqs[pending[0]][0] - (srv.stats()["clock"] - start)))
+    res = {k: srv.result(i) for k, i in ids.items()}
+    wall = max(r["t_done"] for r in res.values()) - start
+    st = srv.stats()
+    srv.close()
+    lat = np.array([res[k]["t_done"] - start - arrival[k] for k in res])
+    conv = sum(r["status"] == cp.POINT_CONVERGED for r in res.values())
+    served = {"points": n_req, "converged": int(conv), "capped": int(n_req - conv),
+              "points_per_s": n_req / wall, "wall_s": wall,
+              "latency_p50_s": float(np.percentile(lat, 50)),
+              "latency_p99_s": float(np.percentile(lat, 99)),
+              "mean_steps": float(np.mean([r["steps"] for r in res.values()])),
+              "rounds": st["rounds"], "batch_rebuilds": st["rebuilds"]}
+    print(f"# served: {served}", file=sys.stderr, flush=True)
+    # plain path: one point at a time, in arrival order, same rule
+    t_clock = 0.0
+    lat_p = []
+    same = True
+    t0 = time.perf_counter()
+    for k in range(n_req):
+        arr, d = reqs[k]
+        t_clock = max(t_clock, arr)
+        p = cp.params(**d, jacobi_mode=cp.JACOBI_RESIDUAL, jacobi_tol=1e-6,
+                      jacobi_tol_init=1e-6, jacobi_max_sweeps=500, jacobi_check_every=50)
+        s0 = time.perf_counter()
+        r = cp.Run(p)
+        r.init_sim()
+        t = p.dt
+        steps, prev, have, ws = 0, 0.0, False, [0, 0]
+        status = cp.POINT_CAPPED
+        while steps < max_steps:
+            r.step(t)
+            t += p.dt
+            steps += 1
+            if steps % window == 0:
+                tot = r.probe_tally()["total_probe"]
+                ie = (tot[0] - ws[0]) / (window * p.dt)
+                ii = (tot[1] - ws[1]) / (window * p.dt)
+                ws = [int(tot[0]), int(tot[1])]
+                if have and converged_rule(prev, ie, ii, rel):
+                    status = cp.POINT_CONVERGED
+                    break
+                prev, have = ii - ie, True
+        r.synchronize()
+        tot = r.probe_tally()["total_probe"]
+        r.close()
+        t_clock += time.perf_counter() - s0
+        lat_p.append(t_clock - arr)
+        same &= (steps == res[k]["steps"] and status == res[k]["status"]
+                 and [int(v) for v in tot] == res[k]["total_probe"])
+    lat_p = np.array(lat_p)
+    plain = {"points_per_s": n_req / max(t_clock, 1e-9),
+             "latency_p50_s": float(np.percentile(lat_p, 50)),
+             "latency_p99_s": float(np.percentile(lat_p, 99)),
+             "busy_s": time.perf_counter() - t0}
+    return {"workload": f"{n_req} I-V points, Poisson arrivals {rate:g}/s, ncx~U{{128..1024}}, "
+                        f"ncy~U{{64..512}}, ppc~U{{20..200}}, phi_p~U[-10,2]; converged when "
+                        f"the net probe current changes < {rel:g} of the gross current "
+                        f"between {window}-step windows (cap {max_steps} steps)",
+            "served": served, "plain_one_at_a_time": plain,
+            "points_per_s_vs_plain": served["points_per_s"] / plain["points_per_s"],
+            "results_identical_to_plain": bool(same)}
+
 
 def main():
     ap = argparse.ArgumentParser()
@@ -91,6 +262,11 @@ def main():
     ap.add_argument("--cpu-max-steps", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-timing", action="store_true")
+    ap.add_argument("--batched-runs", type=int, default=256)
+    ap.add_argument("--batched-steps", type=int, default=10)
+    ap.add_argument("--serve-requests", type=int, default=12)
+    ap.add_argument("--serve-rate", type=float, default=50.0)
+    ap.add_argument("--serve-max-steps", type=int, default=2000)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first copy this many MiB device-to-device with torch: a plain copy "
                          "kernel of known size that scales profiler byte counters")
@@ -211,6 +387,12 @@ def main():
     out["probe"] = {"total_absorbed": [int(v) for v in tally["total_probe"]],
                     "charge_collected": tally["charge"]}
     run.close()
+    if a.batched_runs > 0:
+        out["batched"] = batched_leg(cp, a.batched_runs, a.batched_steps, 3, 20261203)
+        print(f"# batched: {out['batched']}", file=sys.stderr, flush=True)
+    if a.serve_requests > 0:
+        out["serving"] = serving_leg(cp, a.serve_requests, a.serve_rate, a.serve_max_steps,
+                                     20261203)
     if not a.no_cpu and a.cpu_max_steps > 0:
         print("# cpu baseline ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(cfg, a.cpu_seconds, a.cpu_max_steps)

@@ -22,7 +22,7 @@ CUPIC_OK = 0
 ERRORS = {-1: "ARG", -2: "HIP", -3: "ALLOC", -4: "CAPACITY", -5: "STATE", -6: "NUMERIC",
           -7: "IO"}
 
-JACOBI_REFERENCE, JACOBI_RESIDUAL, JACOBI_REFINE = 0, 1, 2
+JACOBI_REFERENCE, JACOBI_RESIDUAL, JACOBI_REFINE, POISSON_DIRECT = 0, 1, 2, 3
 QUIRK_Q7_REFERENCE_KICK = 1
 MESH_RHO, MESH_PHI, MESH_EX, MESH_EY, MESH_CHARGE = 0, 1, 2, 3, 4
 ELECTRONS, IONS = 0, 1
@@ -56,6 +56,36 @@ class Params(C.Structure):
         p = Params()
         C.pointer(p)[0] = self
         return p
+
+
+class ServerConfig(C.Structure):
+    """Mirror of cupic_server_config (include/cupic.h)."""
+    _fields_ = [
+        ("hbm_budget", C.c_int64), ("max_active", C.c_int32), ("round_steps", C.c_int32),
+        ("window_steps", C.c_int32), ("max_steps", C.c_int32), ("rel_change", C.c_double),
+        ("use_graph", C.c_int32), ("jacobi_mode", C.c_int32),
+        ("jacobi_max_sweeps", C.c_int32), ("jacobi_check_every", C.c_int32),
+        ("jacobi_tol", C.c_double),
+    ]
+
+
+class Point(C.Structure):
+    """Mirror of cupic_point (include/cupic.h)."""
+    _fields_ = [
+        ("id", C.c_int64), ("status", C.c_int32), ("error", C.c_int32),
+        ("phi_p", C.c_double), ("footprint", C.c_int64),
+        ("t_submit", C.c_double), ("t_admit", C.c_double), ("t_done", C.c_double),
+        ("steps", C.c_int64), ("windows", C.c_int64),
+        ("current", C.c_double * 2), ("rel_change", C.c_double),
+        ("total_probe", C.c_int64 * 2), ("particles", C.c_int64 * 2),
+    ]
+
+    def as_dict(self):
+        return {k: (list(getattr(self, k)) if k in ("current", "total_probe", "particles")
+                    else getattr(self, k)) for k, _ in self._fields_}
+
+
+POINT_QUEUED, POINT_RUNNING, POINT_CONVERGED, POINT_CAPPED, POINT_FAILED = 0, 1, 2, 3, 4
 
 
 def _load():
@@ -102,6 +132,15 @@ def _load():
         "cupic_mesh_snapshot": [vp, C.c_int, C.c_char_p],
         "cupic_batch_create": [C.POINTER(vp), C.c_int32, C.c_int32, C.POINTER(vp)],
         "cupic_batch_step": [vp, C.c_int32],
+        "cupic_batch_launch": [vp, C.c_int32],
+        "cupic_batch_wait": [vp],
+        "cupic_server_config_default": [C.POINTER(ServerConfig)],
+        "cupic_server_create": [C.POINTER(ServerConfig), C.POINTER(vp)],
+        "cupic_server_submit": [vp, P, i64p],
+        "cupic_server_poll": [vp, i32p, i32p],
+        "cupic_server_result": [vp, C.c_int64, C.POINTER(Point)],
+        "cupic_server_stats": [vp, i64p, i64p, i64p, dp],
+        "cupic_server_destroy": [vp],
         "cupic_batch_destroy": [vp],
         "cupic_timing_enable": [vp, C.c_int32],
         "cupic_timing_read": [vp, dp, i64p, C.c_int32],
@@ -118,6 +157,8 @@ def _load():
     lib.cupic_last_error.restype = C.c_char_p
     lib.cupic_batch_last_error.argtypes = [vp]
     lib.cupic_batch_last_error.restype = C.c_char_p
+    lib.cupic_server_last_error.argtypes = [vp]
+    lib.cupic_server_last_error.restype = C.c_char_p
     lib.cupic_timing_names.argtypes = []
     lib.cupic_timing_names.restype = C.c_char_p
     return lib
@@ -126,9 +167,10 @@ def _load():
 lib = _load()
 
 
-def _check(code, handle=None, batch=False):
+def _check(code, handle=None, batch=False, server=False):
     if code != CUPIC_OK:
-        fn = lib.cupic_batch_last_error if batch else lib.cupic_last_error
+        fn = (lib.cupic_server_last_error if server else
+              lib.cupic_batch_last_error if batch else lib.cupic_last_error)
         msg = fn(handle).decode(errors="replace")
         raise CupicError(code, msg)
 
@@ -336,9 +378,73 @@ class Batch:
     def step(self, n=1):
         _check(lib.cupic_batch_step(self.h, n), self.h, batch=True)
 
+    def launch(self, n=1):
+        _check(lib.cupic_batch_launch(self.h, n), self.h, batch=True)
+
+    def wait(self):
+        _check(lib.cupic_batch_wait(self.h), self.h, batch=True)
+
     def close(self):
         if self.h:
             lib.cupic_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def server_config(**kw):
+    """Server defaults (include/cupic.h cupic_server_config_default), overridden
+    by keyword."""
+    c = ServerConfig()
+    _check(lib.cupic_server_config_default(C.byref(c)))
+    for k, v in kw.items():
+        if not hasattr(c, k):
+            raise KeyError(k)
+        setattr(c, k, v)
+    return c
+
+
+class Server:
+    """Continuous batching of I-V sweep points (one request = one bias point):
+    resident runs advance together in batched rounds until their probe current
+    is steady; requests are admitted first come first served by slots and
+    device-memory budget."""
+
+    def __init__(self, cfg=None, **kw):
+        self.cfg = cfg if cfg is not None else server_config(**kw)
+        self.h = C.c_void_p()
+        _check(lib.cupic_server_create(C.byref(self.cfg), C.byref(self.h)))
+
+    def submit(self, p):
+        i = C.c_int64()
+        _check(lib.cupic_server_submit(self.h, C.byref(p), C.byref(i)), self.h, server=True)
+        return i.value
+
+    def poll(self):
+        """One round; returns (resident runs, queued requests)."""
+        a, q = C.c_int32(), C.c_int32()
+        _check(lib.cupic_server_poll(self.h, C.byref(a), C.byref(q)), self.h, server=True)
+        return a.value, q.value
+
+    def result(self, i):
+        pt = Point()
+        _check(lib.cupic_server_result(self.h, i, C.byref(pt)), self.h, server=True)
+        return pt.as_dict()
+
+    def stats(self):
+        r, b, u = C.c_int64(), C.c_int64(), C.c_int64()
+        t = C.c_double()
+        _check(lib.cupic_server_stats(self.h, C.byref(r), C.byref(b), C.byref(u), C.byref(t)),
+               self.h, server=True)
+        return dict(rounds=r.value, rebuilds=b.value, hbm_used=u.value, clock=t.value)
+
+    def close(self):
+        if self.h:
+            lib.cupic_server_destroy(self.h)
             self.h = C.c_void_p()
 
     def __del__(self):

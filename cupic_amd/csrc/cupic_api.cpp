@@ -109,6 +109,18 @@ int shape_blocks(const RunDev &h, Shape s)
     case kShapeInit: return 2 * ((h.n_init + kBlock - 1) / kBlock);
     case kShapeJacBlk:
         return ((h.ncx + kJacTX - 1) / kJacTX) * ((h.ncy - 1 + kJacTY - 1) / kJacTY);
+    case kShapeDftInit:
+        if (!h.dft_f) return 0;
+        return (int)(((int64_t)h.ncx * 2 * h.nk + kBlock - 1) / kBlock) + (h.nk + kBlock - 1) / kBlock;
+    case kShapeDftFwd:
+        if (!h.dft_f) return 0;
+        return ((h.ncy + 1 + kGT - 1) / kGT) * ((2 * h.nk + kGT - 1) / kGT);
+    case kShapeDftTri:
+        if (!h.dft_f) return 0;
+        return (2 * h.nk + kBlock - 1) / kBlock;
+    case kShapeDftInv:
+        if (!h.dft_f) return 0;
+        return ((h.ncy - 1 + kGT - 1) / kGT) * ((h.ncx + kGT - 1) / kGT);
     default: return 0;
     }
 }
@@ -191,6 +203,15 @@ void seq_solve(const Target &T, int use_args, int mode, double tol, int min_swee
     if (mode == 0) {
         for (int s = 0; s < max_sweeps; s++) launch_jacobi_sweep(N, s);
         launch_solver_end(T.L(kShapeRun), 0, max_sweeps);
+        return;
+    }
+    if (mode == 3) {
+        // direct FP64 solve of the same system, then its FP64 residual
+        launch_dft_fwd(T.L(kShapeDftFwd));
+        launch_dft_tri(T.L(kShapeDftTri));
+        launch_dft_inv(T.L(kShapeDftInv));
+        launch_residual(N, 0, 2, 0);
+        launch_solver_end(T.L(kShapeRun), 0, 0);
         return;
     }
     Launch B = T.L(kShapeJacBlk);
@@ -348,8 +369,8 @@ int validate(const cupic_params *p, std::string *err)
         return set_err(err, CUPIC_ERR_ARG, "ds, dt and n must be positive");
     if (!(p->mi > 0) || !(p->kti > 0))
         return set_err(err, CUPIC_ERR_ARG, "mi and kti must be positive");
-    if (p->jacobi_mode < 0 || p->jacobi_mode > 2)
-        return set_err(err, CUPIC_ERR_ARG, "jacobi_mode must be 0, 1 or 2");
+    if (p->jacobi_mode < 0 || p->jacobi_mode > 3)
+        return set_err(err, CUPIC_ERR_ARG, "jacobi_mode must be 0, 1, 2 or 3");
     if (p->jacobi_max_sweeps < 1 || p->jacobi_max_sweeps > 1000000)
         return set_err(err, CUPIC_ERR_ARG, "jacobi_max_sweeps out of range");
     double ncells = (double)p->ncx * p->ncy;
@@ -523,6 +544,10 @@ int cupic_run_footprint(const cupic_params *pin, int64_t capacity_hint, int64_t 
     b += 2 * z.far_cap * (int64_t)sizeof(FarRec) + 2 * z.max_inject * 8;
     b += z.nnodes * 60;                          // grids
     b += (int64_t)(p.jacobi_max_sweeps + 2) * 12 * kRedSlots;
+    if (p.jacobi_mode == 3) {
+        int64_t W = 2 * (p.ncx / 2 + 1);
+        b += 8 * (2 * (int64_t)p.ncx * W + (int64_t)(p.ncy - 1) * (W / 2) + (int64_t)(p.ncy + 1) * W);
+    }
     *bytes = b;
     return CUPIC_OK;
 }
@@ -613,6 +638,14 @@ int cupic_run_create(const cupic_params *pin, int64_t capacity_hint, cupic_run *
     h.E = dev_alloc<float2>(r, z.nnodes, &st);
     h.sweep_max = dev_alloc<unsigned int>(r, ((size_t)p.jacobi_max_sweeps + 1) * kRedSlots, &st);
     h.resid = dev_alloc<unsigned long long>(r, ((size_t)p.jacobi_max_sweeps + 2) * kRedSlots, &st);
+    if (p.jacobi_mode == 3) {
+        h.nk = p.ncx / 2 + 1;
+        size_t W = 2 * (size_t)h.nk;
+        h.dft_f = dev_alloc<double>(r, (size_t)p.ncx * W, &st);
+        h.dft_g = dev_alloc<double>(r, (size_t)p.ncx * W, &st);
+        h.dft_t = dev_alloc<double>(r, (size_t)(p.ncy - 1) * h.nk, &st);
+        h.dft_z = dev_alloc<double>(r, (size_t)(p.ncy + 1) * W, &st);
+    }
     h.st = dev_alloc<RunState>(r, 1, &st);
     r->d_rd = dev_alloc<RunDev>(r, 1, &st);
     r->d_plan = dev_alloc<int>(r, (size_t)kShapeCount * 2, &st);
@@ -636,6 +669,13 @@ int cupic_run_create(const cupic_params *pin, int64_t capacity_hint, cupic_run *
         hipDeviceSynchronize() != hipSuccess) {
         cupic_run_destroy(r);
         return set_err(nullptr, CUPIC_ERR_HIP, "run setup failed");
+    }
+    if (h.dft_f) {
+        launch_dft_init(target_of(r).L(kShapeDftInit));
+        if (hipStreamSynchronize(r->stream) != hipSuccess) {
+            cupic_run_destroy(r);
+            return set_err(nullptr, CUPIC_ERR_HIP, "direct-solve tables failed");
+        }
     }
     *out = r;
     return CUPIC_OK;
@@ -943,7 +983,7 @@ int cupic_get_mesh(cupic_run *r, int which, void *host, int64_t count)
     }
     case CUPIC_MESH_PHI: {
         std::vector<double> v(nn);
-        if (r->p.jacobi_mode == 2) {
+        if (r->p.jacobi_mode >= 2) {
             HIPTRY(&r->err, hipMemcpy(v.data(), h.phi64, nn * 8, hipMemcpyDeviceToHost));
         } else {
             std::vector<float> f(nn);
@@ -1215,9 +1255,9 @@ int cupic_batch_create(cupic_run *const *runs, int32_t nruns, int32_t use_graph,
     return CUPIC_OK;
 }
 
-int cupic_batch_step(cupic_batch *b, int32_t nsteps)
+int cupic_batch_launch(cupic_batch *b, int32_t nsteps)
 {
-    if (!b) return set_err(nullptr, CUPIC_ERR_ARG, "null batch");
+    if (!b || nsteps < 0) return set_err(nullptr, CUPIC_ERR_ARG, "bad argument");
     Target T = batch_target(b);
     for (int k = 0; k < nsteps; k++) {
         if (b->use_graph) {
@@ -1229,12 +1269,24 @@ int cupic_batch_step(cupic_batch *b, int32_t nsteps)
     hipError_t e = hipGetLastError();
     if (e != hipSuccess)
         return set_err(&b->err, CUPIC_ERR_HIP, std::string("batch step: ") + hipGetErrorString(e));
-    HIPTRY(&b->err, hipStreamSynchronize(b->stream));
     for (cupic_run *r : b->runs) {
-        r->t_next += nsteps * r->p.dt;
+        for (int k = 0; k < nsteps; k++) r->t_next += r->p.dt;   // as the device clock
         r->acc_valid = true;
     }
     return CUPIC_OK;
+}
+
+int cupic_batch_wait(cupic_batch *b)
+{
+    if (!b) return set_err(nullptr, CUPIC_ERR_ARG, "null batch");
+    HIPTRY(&b->err, hipStreamSynchronize(b->stream));
+    return CUPIC_OK;
+}
+
+int cupic_batch_step(cupic_batch *b, int32_t nsteps)
+{
+    int st = cupic_batch_launch(b, nsteps);
+    return st ? st : cupic_batch_wait(b);
 }
 
 int cupic_batch_destroy(cupic_batch *b)

@@ -29,6 +29,8 @@ constexpr int kCmax = 16;              // max cells per wave task
 constexpr int kTileW = kCmax + 3;      // deposit tile width (node columns)
 constexpr int kScanCells = 1024;       // cells per scan block
 constexpr int kRedSlots = 16;         // atomic slots per reduced quantity
+constexpr int kGT = 64;        // direct solve: GEMM output tile
+constexpr int kGK = 16;        // direct solve: GEMM k tile
 constexpr int kJacRows = 16;           // blocked Jacobi: rows per wave
 constexpr int kJacH = 11;              // halo; at most kJacH - 1 sweeps per launch
 constexpr int kJacTX = 64 - 2 * kJacH;                   // tile columns (42)
@@ -127,6 +129,11 @@ struct RunDev {
     float2 *E;                      // nnodes
     unsigned int *sweep_max;        // (max_sweeps_alloc + 1) * kRedSlots (float bits)
     unsigned long long *resid;      // (max_sweeps_alloc + 2) * kRedSlots (double bits)
+    int nk;                         // mode 3: wavenumbers 0 .. ncx/2 (K = ncx/2 + 1)
+    double *dft_f;                  // mode 3: [ncx][2K] forward basis
+    double *dft_g;                  // mode 3: [2K][ncx] inverse basis
+    double *dft_t;                  // mode 3: [ncy-1][K] Thomas multipliers
+    double *dft_z;                  // mode 3: [ncy+1][2K] transformed rows
     RunState *st;
 };
 

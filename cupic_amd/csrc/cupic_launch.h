@@ -31,6 +31,10 @@ enum Shape : int {
     kShapeRows,        // 2 * ncy
     kShapeInit,        // 2 * ceil(n_init / 256)
     kShapeJacBlk,      // ceil(ncx / kJacTX) * ceil((ncy - 1) / kJacTY)
+    kShapeDftInit,     // ceil(ncx * 2K / 256) + ceil(K / 256)
+    kShapeDftFwd,      // ceil((ncy + 1) / kGT) * ceil(2K / kGT)
+    kShapeDftTri,      // ceil(2K / 256)
+    kShapeDftInv,      // ceil((ncy - 1) / kGT) * ceil(ncx / kGT)
     kShapeCount
 };
 
@@ -48,6 +52,12 @@ void launch_ir_update(const Launch &L, int par);
 void launch_residual(const Launch &L, int c, int mode, int par);
 void launch_solver_end(const Launch &L, int last_check, int par);
 void launch_field(const Launch &L);
+
+// direct solve (mode 3)
+void launch_dft_init(const Launch &L);
+void launch_dft_fwd(const Launch &L);
+void launch_dft_tri(const Launch &L);
+void launch_dft_inv(const Launch &L);
 
 // particles
 void launch_create(const Launch &L);

@@ -362,7 +362,7 @@ __global__ __launch_bounds__(kBlock) void k_solver_end(const RunDev *rds, const 
     RUN_PROLOGUE();
     if (threadIdx.x != 0) return;
     if (!st->solve_done) {
-        int s = st->solve_max;
+        int s = st->solve_mode == 3 ? 0 : st->solve_max;
         st->sweeps_done = s;
         if (st->solve_mode == 0)
             st->last_resid = s > 0 ? (double)read_sweep_max(R, s - 1) : 0.0;
@@ -378,7 +378,7 @@ __global__ __launch_bounds__(kBlock) void k_solver_end(const RunDev *rds, const 
 
 // E = -grad phi (src/mesh.cu:413-468): central differences with periodic x
 // on interior rows; one-sided Ey and Ex = 0 on rows 0 and ncy.  Evaluated in
-// FP64 from the FP32 (modes 0/1) or FP64 (mode 2) potential.
+// FP64 from the FP32 (modes 0/1) or FP64 (modes 2/3) potential.
 __global__ __launch_bounds__(kBlock) void k_field(const RunDev *rds, const int *start,
                                                   int nruns)
 {
@@ -388,7 +388,7 @@ __global__ __launch_bounds__(kBlock) void k_field(const RunDev *rds, const int *
     int ncx = R.ncx;
     int j = k / ncx;
     int i = k - j * ncx;
-    bool m2 = (st->solve_mode == 2);
+    bool m2 = (st->solve_mode >= 2);
     const float *p32 = R.phi32[st->phi_cur];
     const double *p64 = R.phi64;
     auto P = [&](int q) -> double { return m2 ? p64[q] : (double)p32[q]; };

@@ -39,6 +39,8 @@ extern "C" {
 #define CUPIC_JACOBI_REFERENCE 0  /* max|dphi| < tol after >= min sweeps (src/mesh.cu:94-114) */
 #define CUPIC_JACOBI_RESIDUAL 1   /* FP32 sweeps, FP64 residual checked every check_every     */
 #define CUPIC_JACOBI_REFINE 2     /* FP64 residual + FP32 correction solve (refinement)        */
+#define CUPIC_POISSON_DIRECT 3    /* the system Jacobi converges to, solved directly in FP64:
+                                     DFT along periodic x, tridiagonal solve per wavenumber  */
 
 /* quirk flags */
 #define CUPIC_QUIRK_Q7_REFERENCE_KICK 1u /* injection half-kick sign as committed (src/cc.cu:243-244) */
@@ -182,8 +184,70 @@ int cupic_mesh_snapshot(cupic_run *r, int which, const char *path);
 int cupic_batch_create(cupic_run *const *runs, int32_t nruns, int32_t use_graph,
                        cupic_batch **out);
 int cupic_batch_step(cupic_batch *b, int32_t nsteps);
+/* cupic_batch_step in two halves: enqueue nsteps steps and return at once;
+ * wait for them.  Nothing else may touch the batch's runs in between. */
+int cupic_batch_launch(cupic_batch *b, int32_t nsteps);
+int cupic_batch_wait(cupic_batch *b);
 int cupic_batch_destroy(cupic_batch *b);
 const char *cupic_batch_last_error(const cupic_batch *b);
+
+/* ---- serving: continuous batching of I-V sweep points --------------------- */
+
+/* No reference counterpart (the reference runs one configuration per
+ * process).  Each request is one bias point with its own parameters; admitted
+ * runs stay resident and advance together in batched rounds until their
+ * probe current is steady.  Convergence: at every window_steps steps of a
+ * run, the change of the window-mean net probe current (ions - electrons)
+ * since the previous window, relative to the gross current (ions +
+ * electrons), is below rel_change. */
+typedef struct cupic_server_config {
+    int64_t hbm_budget;        /* bytes of device memory for resident runs     */
+    int32_t max_active;        /* resident runs at most                        */
+    int32_t round_steps;       /* steps per batched round (divides window)     */
+    int32_t window_steps;      /* probe-current window                         */
+    int32_t max_steps;         /* cap per point (0: none)                      */
+    double rel_change;         /* convergence threshold                        */
+    int32_t use_graph;
+    int32_t jacobi_mode;       /* applied to every request                     */
+    int32_t jacobi_max_sweeps;
+    int32_t jacobi_check_every;
+    double jacobi_tol;
+} cupic_server_config;
+
+#define CUPIC_POINT_QUEUED 0
+#define CUPIC_POINT_RUNNING 1
+#define CUPIC_POINT_CONVERGED 2
+#define CUPIC_POINT_CAPPED 3     /* reached max_steps first                    */
+#define CUPIC_POINT_FAILED 4     /* error holds the status                     */
+
+typedef struct cupic_point {
+    int64_t id;
+    int32_t status;
+    int32_t error;
+    double phi_p;
+    int64_t footprint;         /* device bytes while resident                 */
+    double t_submit, t_admit, t_done;   /* server clock, seconds (-1: not yet) */
+    int64_t steps;
+    int64_t windows;
+    double current[2];         /* last window: absorbed per unit time, e-, ions */
+    double rel_change;         /* last window-to-window change                 */
+    int64_t total_probe[2];
+    int64_t particles[2];
+} cupic_point;
+
+typedef struct cupic_server cupic_server;
+
+int cupic_server_config_default(cupic_server_config *cfg);
+int cupic_server_create(const cupic_server_config *cfg, cupic_server **out);
+/* queues a point; the Jacobi settings of the config replace those of p */
+int cupic_server_submit(cupic_server *s, const cupic_params *p, int64_t *id);
+/* one round: admit, advance every resident run by round_steps, retire */
+int cupic_server_poll(cupic_server *s, int32_t *n_active, int32_t *n_queued);
+int cupic_server_result(cupic_server *s, int64_t id, cupic_point *out);
+int cupic_server_stats(cupic_server *s, int64_t *rounds, int64_t *rebuilds, int64_t *hbm_used,
+                       double *clock);
+int cupic_server_destroy(cupic_server *s);
+const char *cupic_server_last_error(const cupic_server *s);
 
 /* ---- timing ------------------------------------------------------------- */
 

@@ -2,7 +2,8 @@
 the C-ABI: initial load and sort, offsets, the field of a given potential,
 the half-step stagger, one push/absorb/inject/sort step and its fixed-point
 charge (all bit-exact), and the potential that initialize_mesh solves from
-the initial charge (FP64 iterative refinement, within 1e-8 relative)."""
+the initial charge (FP64 iterative refinement within 1e-8 relative, the
+direct FP64 solve within 1e-11)."""
 import glob
 import os
 import sys
@@ -91,11 +92,15 @@ def test_golden_step_chain(dev, path):
     run.close()
 
 
+@pytest.mark.parametrize("mode,tol", [(cp.JACOBI_REFINE, 1e-8), (cp.POISSON_DIRECT, 1e-11)],
+                         ids=["refine", "direct"])
 @pytest.mark.parametrize("path", FILES, ids=[os.path.basename(f) for f in FILES])
-def test_golden_charge_and_potential(dev, path):
+def test_golden_charge_and_potential(dev, path, mode, tol):
+    """Iterative refinement to an FP64 residual of 1e-11, or the direct FP64
+    solve of the same system, against the fixture's converged potential."""
     d = load(path)
-    say("golden potential", os.path.basename(path))
-    p = params_of(d, jacobi_mode=cp.JACOBI_REFINE, jacobi_max_sweeps=20000,
+    say("golden potential", os.path.basename(path), "mode", mode)
+    p = params_of(d, jacobi_mode=mode, jacobi_max_sweeps=20000,
                   jacobi_check_every=50, jacobi_tol=1e-11, jacobi_tol_init=1e-11)
     run = cp.Run(p)
     run.create_particles()
@@ -116,5 +121,5 @@ def test_golden_charge_and_potential(dev, path):
     want = d["phi"]
     err = np.abs(phi - want).max() / np.abs(want).max()
     say("  potential relative error", err)
-    assert err < 1e-8
+    assert err < tol
     run.close()

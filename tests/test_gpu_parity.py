@@ -216,14 +216,15 @@ def test_runs_are_deterministic(dev):
             assert np.array_equal(u, v)
 
 
-def test_batch_graph_equals_plain_runs(dev):
+@pytest.mark.parametrize("mode", [cp.JACOBI_RESIDUAL, cp.POISSON_DIRECT], ids=["jacobi", "direct"])
+def test_batch_graph_equals_plain_runs(dev, mode):
     """Alone vs in a mix: runs of different shapes advanced together in one
     captured graph give bit-identical states to each run stepped alone."""
     shapes = [dict(ncx=64, ncy=32, ppc=10), dict(ncx=40, ncy=20, ppc=6, phi_p=1.0, seed=7),
               dict(ncx=96, ncy=16, ppc=12, phi_p=-8.0, seed=9)]
     alone, mixed = [], []
     for kw in shapes:
-        p = small(jacobi_max_sweeps=100, **kw)
+        p = small(jacobi_max_sweeps=100, jacobi_mode=mode, **kw)
         for lst in (alone, mixed):
             r = cp.Run(p)
             r.init_sim()
