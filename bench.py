@@ -80,6 +80,35 @@ def profiled_traffic(kernel):
         pass
     return None, None
 
+def direct_leg(cp, cfg, steps, warmup):
+    """The same cycle with the Poisson system solved directly in FP64 (the
+    converged fixed point of the Jacobi iteration) instead of 500 sweeps."""
+    p = cp.params(**cfg, jacobi_mode=cp.POISSON_DIRECT)
+    run = cp.Run(p)
+    run.init_sim()
+    t = p.dt
+    for _ in range(warmup):
+        run.step(t)
+        t += p.dt
+    run.synchronize()
+    n0 = run.number_of_particles(0) + run.number_of_particles(1)
+    run.timing(True)
+    run.timing_read()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        run.step(t)
+        t += p.dt
+    run.synchronize()
+    wall = time.perf_counter() - t0
+    kern = run.timing_read()
+    n1 = run.number_of_particles(0) + run.number_of_particles(1)
+    _, resid = run.last_solve()
+    run.close()
+    return {"value": 0.5 * (n0 + n1) * steps / wall, "unit": "particle-steps/s",
+            "ms_per_step": 1e3 * wall / steps, "poisson_fp64_residual": resid,
+            "solve_ms_per_launch": kern["solve"][0] / max(kern["solve"][1], 1)}
+
+
 def draw_batched(n_runs, seed):
     """256 independent runs: ncx ~ U{64..512}, ncy ~ U{32..256}, ppc ~ U{20..100},
     bias ~ U[-10, 2] kTe/e; other parameters as the small configuration."""
@@ -172,8 +201,9 @@ def serving_leg(cp, n_req, rate, max_steps, seed):
     import numpy as np
     reqs = draw_requests(n_req, rate, seed)
     window, rnd, rel = 500, 50, 0.005
-    srv = cp.Server(cp.server_config(max_steps=max_steps, window_steps=window,
-                                     round_steps=rnd, rel_change=rel))
+    srv_cfg = cp.server_config(max_steps=max_steps, window_steps=window, round_steps=rnd,
+                               rel_change=rel)
+    srv = cp.Server(srv_cfg)
     start = srv.stats()["clock"]
     pending = list(range(n_req))
     ids = {}
@@ -210,8 +240,10 @@ def serving_leg(cp, n_req, rate, max_steps, seed):
     for k in range(n_req):
         arr, d = reqs[k]
         t_clock = max(t_clock, arr)
-        p = cp.params(**d, jacobi_mode=cp.JACOBI_RESIDUAL, jacobi_tol=1e-6,
-                      jacobi_tol_init=1e-6, jacobi_max_sweeps=500, jacobi_check_every=50)
+        c = srv_cfg
+        p = cp.params(**d, jacobi_mode=c.jacobi_mode, jacobi_tol=c.jacobi_tol,
+                      jacobi_tol_init=c.jacobi_tol, jacobi_max_sweeps=c.jacobi_max_sweeps,
+                      jacobi_check_every=c.jacobi_check_every)
         s0 = time.perf_counter()
         r = cp.Run(p)
         r.init_sim()
@@ -246,7 +278,8 @@ def serving_leg(cp, n_req, rate, max_steps, seed):
     return {"workload": f"{n_req} I-V points, Poisson arrivals {rate:g}/s, ncx~U{{128..1024}}, "
                         f"ncy~U{{64..512}}, ppc~U{{20..200}}, phi_p~U[-10,2]; converged when "
                         f"the net probe current changes < {rel:g} of the gross current "
-                        f"between {window}-step windows (cap {max_steps} steps)",
+                        f"between {window}-step windows (cap {max_steps} steps); direct FP64 "
+                        f"Poisson solve",
             "served": served, "plain_one_at_a_time": plain,
             "points_per_s_vs_plain": served["points_per_s"] / plain["points_per_s"],
             "results_identical_to_plain": bool(same)}
@@ -262,6 +295,7 @@ def main():
     ap.add_argument("--cpu-max-steps", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-timing", action="store_true")
+    ap.add_argument("--no-direct", action="store_true")
     ap.add_argument("--batched-runs", type=int, default=256)
     ap.add_argument("--batched-steps", type=int, default=10)
     ap.add_argument("--serve-requests", type=int, default=12)
@@ -387,6 +421,9 @@ def main():
     out["probe"] = {"total_absorbed": [int(v) for v in tally["total_probe"]],
                     "charge_collected": tally["charge"]}
     run.close()
+    if not a.no_direct:
+        out["plain_direct_poisson"] = direct_leg(cp, cfg, a.steps, a.warmup)
+        print(f"# direct: {out['plain_direct_poisson']}", file=sys.stderr, flush=True)
     if a.batched_runs > 0:
         out["batched"] = batched_leg(cp, a.batched_runs, a.batched_steps, 3, 20261203)
         print(f"# batched: {out['batched']}", file=sys.stderr, flush=True)

@@ -195,7 +195,10 @@ int cupic_server_config_default(cupic_server_config *c)
     c->max_steps = 20000;
     c->rel_change = 0.005;
     c->use_graph = 1;
-    c->jacobi_mode = CUPIC_JACOBI_RESIDUAL;
+    // a point must reach a physically steady state: the capped Jacobi solve
+    // cannot keep grids of hundreds of rows quasi-neutral over thousands of
+    // steps, the direct solve of the same system can
+    c->jacobi_mode = CUPIC_POISSON_DIRECT;
     c->jacobi_max_sweeps = 500;
     c->jacobi_check_every = 50;
     c->jacobi_tol = 1e-6;

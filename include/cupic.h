@@ -237,6 +237,8 @@ typedef struct cupic_point {
 
 typedef struct cupic_server cupic_server;
 
+/* defaults: 200 GiB, 256 runs, rounds of 50 steps, 500-step windows, cap
+ * 20000 steps, rel_change 0.005, graphs on, CUPIC_POISSON_DIRECT */
 int cupic_server_config_default(cupic_server_config *cfg);
 int cupic_server_create(const cupic_server_config *cfg, cupic_server **out);
 /* queues a point; the Jacobi settings of the config replace those of p */
