@@ -80,10 +80,10 @@ def profiled_traffic(kernel):
         pass
     return None, None
 
-def direct_leg(cp, cfg, steps, warmup):
+def direct_leg(cp, cfg, steps, warmup, tile):
     """The same cycle with the Poisson system solved directly in FP64 (the
     converged fixed point of the Jacobi iteration) instead of 500 sweeps."""
-    p = cp.params(**cfg, jacobi_mode=cp.POISSON_DIRECT)
+    p = cp.params(**cfg, jacobi_mode=cp.POISSON_DIRECT, tile_cells=tile)
     run = cp.Run(p)
     run.init_sim()
     t = p.dt
@@ -124,7 +124,9 @@ def draw_batched(n_runs, seed):
 
 def batched_leg(cp, n_runs, steps, seq_steps, seed):
     """One HIP-graph step advances all runs; baseline: the same runs stepped
-    one after another through the plain path (cupic_step per run)."""
+    one after another through the plain path (cupic_step per run).  The tile
+    (cells per wave strip) is tuned in place on the batch itself: each
+    candidate is set on every run and timed over 2 batched steps."""
     draws = draw_batched(n_runs, seed)
     runs = []
     for d in draws:
@@ -137,18 +139,37 @@ def batched_leg(cp, n_runs, steps, seq_steps, seed):
         r.synchronize()
     npart = sum(r.number_of_particles(0) + r.number_of_particles(1) for r in runs)
     print(f"# batched: {n_runs} runs, {npart} particles", file=sys.stderr, flush=True)
+    tune = {}
+    done = 0
+    for tile in (0, 2, 4, 6, 8, 12, 16):
+        for r in runs:
+            r.set_tile(tile)
+        b = cp.Batch(runs, graph=True)
+        b.step(1)
+        t0 = time.perf_counter()
+        b.step(2)
+        tune[tile] = 1e3 * (time.perf_counter() - t0) / 2
+        b.close()
+        done += 3
+    tile = min(tune, key=tune.get)
+    print(f"# batched: tile tuning (ms per batched step) {tune} -> {tile}", file=sys.stderr,
+          flush=True)
+    for r in runs:
+        r.set_tile(tile)
     b = cp.Batch(runs, graph=True)
-    b.step(2)
+    b.step(1)
+    done += 1
     t0 = time.perf_counter()
     b.step(steps)
     wall_b = time.perf_counter() - t0
     b.close()
+    done += steps
     # the runs continue on the plain path, one after another, from the time
     # the batch reached (t accumulated as t += dt, like the device clock)
     t_next = []
     for r in runs:
         t = 0.0
-        for _ in range(3 + steps):
+        for _ in range(1 + done):
             t += r.p.dt
         r.step(t)                     # warm-up of the plain path
         r.synchronize()
@@ -165,6 +186,7 @@ def batched_leg(cp, n_runs, steps, seq_steps, seed):
     rs_b = n_runs * steps / wall_b
     rs_s = n_runs * seq_steps / wall_s
     return {"runs": n_runs, "particles": int(npart), "steps": steps,
+            "tile_cells": tile, "tile_tuning_ms_per_step": tune,
             "ms_per_batched_step": 1e3 * wall_b / steps,
             "run_steps_per_s": rs_b, "particle_steps_per_s": npart * steps / wall_b,
             "sequential_plain_run_steps_per_s": rs_s,
@@ -296,6 +318,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-timing", action="store_true")
     ap.add_argument("--no-direct", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--batched-runs", type=int, default=256)
     ap.add_argument("--batched-steps", type=int, default=10)
     ap.add_argument("--serve-requests", type=int, default=12)
@@ -351,6 +374,11 @@ def main():
     for _ in range(a.warmup):
         run.step(t)
         t += p.dt
+    tile = 0
+    if not a.no_autotune:
+        # in-situ: each candidate tile steps this run (untimed warm-up steps)
+        t, tile, _, ttimes = run.autotune(t, steps=2)
+        print(f"# autotuned tile {tile}: {ttimes}", file=sys.stderr, flush=True)
     run.synchronize()
     print("# warm-up done", file=sys.stderr, flush=True)
     n0 = run.number_of_particles(0) + run.number_of_particles(1)
@@ -392,7 +420,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": workload_name(cfg, p), "grid": [p.ncx, p.ncy],
                    "ppc": cfg["ppc"], "particles": int(n1), "jacobi_mode": "fp64-residual",
-                   "jacobi_sweeps_per_step": S, "device": name},
+                   "jacobi_sweeps_per_step": S, "tile_cells": tile or "auto",
+                   "device": name},
     }
     if kern:
         per = {k: (v[0] / v[1] if v[1] else 0.0) for k, v in kern.items()}
@@ -422,7 +451,7 @@ def main():
                     "charge_collected": tally["charge"]}
     run.close()
     if not a.no_direct:
-        out["plain_direct_poisson"] = direct_leg(cp, cfg, a.steps, a.warmup)
+        out["plain_direct_poisson"] = direct_leg(cp, cfg, a.steps, a.warmup, tile)
         print(f"# direct: {out['plain_direct_poisson']}", file=sys.stderr, flush=True)
     if a.batched_runs > 0:
         out["batched"] = batched_leg(cp, a.batched_runs, a.batched_steps, 3, 20261203)

@@ -45,7 +45,7 @@ class Params(C.Structure):
         ("quirks", C.c_uint32),
         ("jacobi_mode", C.c_int32), ("jacobi_min_sweeps", C.c_int32),
         ("jacobi_max_sweeps", C.c_int32), ("jacobi_check_every", C.c_int32),
-        ("pad0", C.c_int32),
+        ("tile_cells", C.c_int32),
         ("jacobi_tol", C.c_double), ("jacobi_tol_init", C.c_double),
         ("Lx", C.c_double), ("Ly", C.c_double),
         ("dtin_e", C.c_double), ("dtin_i", C.c_double),
@@ -132,6 +132,10 @@ def _load():
         "cupic_mesh_snapshot": [vp, C.c_int, C.c_char_p],
         "cupic_batch_create": [C.POINTER(vp), C.c_int32, C.c_int32, C.POINTER(vp)],
         "cupic_batch_step": [vp, C.c_int32],
+        "cupic_autotune_tile": [P, C.c_int32, i32p, dp, dp],
+        "cupic_autotune_run": [vp, dp, C.c_int32, i32p, dp, dp],
+        "cupic_run_set_tile": [vp, C.c_int32],
+        "cupic_run_params": [vp, P],
         "cupic_batch_launch": [vp, C.c_int32],
         "cupic_batch_wait": [vp],
         "cupic_server_config_default": [C.POINTER(ServerConfig)],
@@ -354,6 +358,23 @@ class Run:
     def mesh_snapshot(self, which, path):
         self._c(lib.cupic_mesh_snapshot(self.h, which, path.encode()))
 
+    def set_tile(self, tile):
+        """Cells per wave strip of the push passes (0: from ppc); speed only."""
+        self._c(lib.cupic_run_set_tile(self.h, tile))
+        self.p.tile_cells = tile
+
+    def autotune(self, t, steps=3):
+        """Tunes the tile on this run by stepping it from time t; returns
+        (next t, tile, ms per step, {tile: ms})."""
+        tt = C.c_double(t)
+        best = C.c_int32()
+        ms = C.c_double()
+        times = (C.c_double * 17)()
+        self._c(lib.cupic_autotune_run(self.h, C.byref(tt), steps, C.byref(best), C.byref(ms),
+                                       times))
+        self.p.tile_cells = best.value
+        return tt.value, best.value, ms.value, {c: times[c] for c in range(17) if times[c] > 0}
+
     def timing(self, on=True):
         self._c(lib.cupic_timing_enable(self.h, 1 if on else 0))
 
@@ -394,6 +415,17 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def autotune_tile(p, steps=5):
+    """Fastest tile_cells for runs shaped like p (push passes timed with HIP
+    events); returns (tile, ms per step, {tile: ms})."""
+    best = C.c_int32()
+    ms = C.c_double()
+    times = (C.c_double * 17)()
+    q = p.copy()
+    _check(lib.cupic_autotune_tile(C.byref(q), steps, C.byref(best), C.byref(ms), times))
+    return best.value, ms.value, {c: times[c] for c in range(17) if times[c] > 0}
 
 
 def server_config(**kw):

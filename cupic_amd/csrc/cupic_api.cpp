@@ -371,6 +371,8 @@ int validate(const cupic_params *p, std::string *err)
         return set_err(err, CUPIC_ERR_ARG, "mi and kti must be positive");
     if (p->jacobi_mode < 0 || p->jacobi_mode > 3)
         return set_err(err, CUPIC_ERR_ARG, "jacobi_mode must be 0, 1, 2 or 3");
+    if (p->tile_cells < 0 || p->tile_cells > kCmax)
+        return set_err(err, CUPIC_ERR_ARG, "tile_cells must be 0 (automatic) or 1..16");
     if (p->jacobi_max_sweeps < 1 || p->jacobi_max_sweeps > 1000000)
         return set_err(err, CUPIC_ERR_ARG, "jacobi_max_sweeps out of range");
     double ncells = (double)p->ncx * p->ncy;
@@ -397,7 +399,8 @@ Sizes sizes_of(const cupic_params *p, int64_t capacity_hint)
     z.ncells = (int64_t)p->ncx * p->ncy;
     z.nblk_scan = (z.ncells + kScanCells - 1) / kScanCells;
     double ppc = (double)p->per_row / p->ncx;
-    int C = (int)std::lround(2048.0 / std::max(ppc, 1.0));
+    int C = p->tile_cells > 0 ? p->tile_cells
+                              : (int)std::lround(2048.0 / std::max(ppc, 1.0));
     z.C = std::max(1, std::min(kCmax, C));
     return z;
 }
@@ -528,6 +531,37 @@ int cupic_params_from_input_file(const char *path, cupic_params *p, int32_t *n_i
     if (n_save) *n_save = (int32_t)v[3];
     if (n_fin) *n_fin = (int32_t)v[4];
     return cupic_params_derive(p);
+}
+
+int cupic_run_params(const cupic_run *r, cupic_params *out)
+{
+    if (!r || !out) return set_err(nullptr, CUPIC_ERR_ARG, "bad argument");
+    *out = r->p;
+    return CUPIC_OK;
+}
+
+int cupic_run_set_tile(cupic_run *r, int32_t tile)
+{
+    if (!r) return set_err(nullptr, CUPIC_ERR_ARG, "null run");
+    if (tile < 0 || tile > kCmax) return set_err(&r->err, CUPIC_ERR_ARG, "tile must be 0..16");
+    if (r->moved) return set_err(&r->err, CUPIC_ERR_STATE, "set_tile between mover and cc");
+    cupic_params q = r->p;
+    q.tile_cells = tile;
+    Sizes z = sizes_of(&q, r->h.cap);
+    HIPTRY(&r->err, hipStreamSynchronize(r->stream));
+    r->p.tile_cells = tile;
+    r->h.C = z.C;
+    r->h.tpr = (r->h.ncx + z.C - 1) / z.C;
+    std::vector<int> plan(kShapeCount * 2);
+    for (int s = 0; s < kShapeCount; s++) {
+        r->nblk[s] = shape_blocks(r->h, (Shape)s);
+        plan[2 * s] = 0;
+        plan[2 * s + 1] = r->nblk[s];
+    }
+    HIPTRY(&r->err, hipMemcpy(r->d_rd, &r->h, sizeof(RunDev), hipMemcpyHostToDevice));
+    HIPTRY(&r->err, hipMemcpy(r->d_plan, plan.data(), plan.size() * sizeof(int),
+                              hipMemcpyHostToDevice));
+    return CUPIC_OK;
 }
 
 int cupic_run_footprint(const cupic_params *pin, int64_t capacity_hint, int64_t *bytes)

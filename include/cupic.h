@@ -59,7 +59,9 @@ typedef struct cupic_params {
     int32_t jacobi_min_sweeps;   /* mode 0 minimum (reference: max(nnx,nny)+1)   */
     int32_t jacobi_max_sweeps;   /* cap on sweeps per solve                       */
     int32_t jacobi_check_every;  /* modes 1, 2: sweeps between FP64 residuals    */
-    int32_t pad0;
+    int32_t tile_cells;      /* cells per wave strip in the push passes: 0 picks
+                                round(2048 / ppc), 1..16 fixes it (results do not
+                                depend on it; see cupic_autotune_tile)           */
     double jacobi_tol;       /* per-step tolerance (reference 1e-4, src/main.cu:64) */
     double jacobi_tol_init;  /* init tolerance (reference 1e-3, src/init.cu:254)     */
     /* derived by cupic_params_derive */
@@ -96,6 +98,8 @@ int cupic_params_from_input_file(const char *path, cupic_params *p, int32_t *n_i
  * capacity_hint: particles per species; 0 picks N + headroom. */
 int cupic_run_create(const cupic_params *p, int64_t capacity_hint, cupic_run **out);
 int cupic_run_destroy(cupic_run *r);
+/* the run's parameters (derived fields included) */
+int cupic_run_params(const cupic_run *r, cupic_params *out);
 const char *cupic_last_error(const cupic_run *r);   /* r may be NULL */
 int cupic_synchronize(cupic_run *r);
 /* device bytes this run holds */
@@ -250,6 +254,28 @@ int cupic_server_stats(cupic_server *s, int64_t *rounds, int64_t *rebuilds, int6
                        double *clock);
 int cupic_server_destroy(cupic_server *s);
 const char *cupic_server_last_error(const cupic_server *s);
+
+/* ---- autotuning ---------------------------------------------------------- */
+
+/* No reference counterpart.  Times the two push passes (k_push_rank +
+ * k_scatter, HIP events) of a run with parameters p for each candidate
+ * tile_cells in 1..16 that divides the work sensibly (1, 2, 3, 4, 6, 8, 10,
+ * 12, 16), `steps` steps after 2 warm-up steps each, and returns the fastest.
+ * times[c] (c = 0..16, may be NULL) receives ms per step of candidate c, or
+ * 0 where c was not tried.  The particle states do not depend on the tile,
+ * so the choice changes speed only. */
+int cupic_autotune_tile(const cupic_params *p, int32_t steps, int32_t *best_tile,
+                        double *best_ms, double *times);
+
+/* Changes a run's tile between steps (0 = round(2048 / ppc)).  A batch keeps
+ * the descriptors it was created with: create batches after this call. */
+int cupic_run_set_tile(cupic_run *r, int32_t tile);
+
+/* In-situ tuning: steps the run itself `steps` steps per candidate tile
+ * (after one step to settle each), from *t (advanced as t += dt per step),
+ * and leaves the fastest tile set.  The steps are real steps of the run. */
+int cupic_autotune_run(cupic_run *r, double *t, int32_t steps, int32_t *best_tile,
+                       double *best_ms, double *times);
 
 /* ---- timing ------------------------------------------------------------- */
 

@@ -243,3 +243,56 @@ def test_batch_graph_equals_plain_runs(dev, mode):
             for u, v in zip(ra.particles(s), rm.particles(s)):
                 assert np.array_equal(u, v)
         assert np.array_equal(ra.mesh(cp.MESH_PHI), rm.mesh(cp.MESH_PHI))
+
+
+def test_tile_does_not_change_results(dev):
+    """The push passes' strip width (cells per wave) changes speed only:
+    states, offsets, charge and tallies are bit-identical for every tile."""
+    outs = []
+    for tile in (1, 3, 7, 16):
+        p = small(ncx=45, ncy=12, ppc=9, tile_cells=tile)
+        say("tile", tile)
+        r = cp.Run(p)
+        r.init_sim()
+        t = p.dt
+        for _ in range(4):
+            r.step(t)
+            t += p.dt
+        outs.append(([np.stack(r.particles(s)) for s in (0, 1)],
+                     [r.offsets(s) for s in (0, 1)], r.mesh(cp.MESH_CHARGE),
+                     list(r.probe_tally()["total_probe"])))
+        r.close()
+    for o in outs[1:]:
+        for a, b in zip(o[0], outs[0][0]):
+            assert a.tobytes() == b.tobytes()
+        for a, b in zip(o[1], outs[0][1]):
+            assert np.array_equal(a, b)
+        assert np.array_equal(o[2], outs[0][2]) and o[3] == outs[0][3]
+
+
+def test_autotune_picks_the_fastest_timed_tile(dev):
+    best, ms, times = cp.autotune_tile(small(ncx=128, ncy=64, ppc=20), steps=2)
+    say("autotune", best, ms, times)
+    assert best in times and ms == times[best] == min(times.values())
+    assert set(times) <= {1, 2, 3, 4, 6, 8, 10, 12, 16}
+
+
+def test_in_situ_autotune_keeps_the_trajectory(dev):
+    """Tuning a run in place steps it with changing tiles; the states equal
+    those of a run stepped the same number of steps with a fixed tile."""
+    p = small(ncx=70, ncy=20, ppc=8)
+    a, b = cp.Run(p), cp.Run(p)
+    a.init_sim()
+    b.init_sim()
+    t, best, ms, times = a.autotune(p.dt, steps=1)
+    n = len(times) * 2
+    tb = p.dt
+    for _ in range(n):
+        b.step(tb)
+        tb += p.dt
+    assert t == tb and a.p.tile_cells == best
+    for s in (0, 1):
+        assert np.stack(a.particles(s)).tobytes() == np.stack(b.particles(s)).tobytes()
+    a.step(t)
+    b.step(tb)
+    assert list(a.probe_tally()["total_probe"]) == list(b.probe_tally()["total_probe"])
