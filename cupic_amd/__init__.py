@@ -124,6 +124,7 @@ def _load():
         "cupic_get_particles": [vp, C.c_int, vp, vp, vp, vp, C.c_int64, i64p],
         "cupic_set_particles": [vp, C.c_int, vp, vp, vp, vp, C.c_int64],
         "cupic_get_offsets": [vp, C.c_int, vp],
+        "cupic_density_profile": [vp, C.c_int, vp],
         "cupic_get_mesh": [vp, C.c_int, vp, C.c_int64],
         "cupic_set_phi": [vp, vp, C.c_int64],
         "cupic_get_clock": [vp, dp, C.POINTER(C.c_uint64), i32p],
@@ -327,6 +328,12 @@ class Run:
         o = np.empty(self.p.ncx * self.p.ncy + 1, np.int32)
         self._c(lib.cupic_get_offsets(self.h, species, o.ctypes.data))
         return o
+
+    def density_profile(self, species):
+        """particles per cell row, shape (ncy,)"""
+        a = np.empty(self.p.ncy, np.int64)
+        self._c(lib.cupic_density_profile(self.h, species, a.ctypes.data))
+        return a
 
     def mesh(self, which):
         """A mesh quantity in the reference layout, shape (nny, nnx)."""

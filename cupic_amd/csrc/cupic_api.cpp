@@ -987,6 +987,18 @@ int cupic_set_particles(cupic_run *r, int species, const float *x, const float *
     return write_state(r, s);
 }
 
+int cupic_density_profile(cupic_run *r, int species, int64_t *per_row)
+{
+    if (!r || !per_row || species < 0 || species > 1)
+        return set_err(nullptr, CUPIC_ERR_ARG, "bad argument");
+    std::vector<int32_t> off((size_t)r->h.ncells + 1);
+    int st = cupic_get_offsets(r, species, off.data());
+    if (st) return st;
+    for (int j = 0; j < r->h.ncy; j++)
+        per_row[j] = (int64_t)off[(size_t)(j + 1) * r->h.ncx] - off[(size_t)j * r->h.ncx];
+    return CUPIC_OK;
+}
+
 int cupic_get_offsets(cupic_run *r, int species, int32_t *offsets)
 {
     if (!r || !offsets || species < 0 || species > 1) return set_err(nullptr, CUPIC_ERR_ARG, "bad argument");

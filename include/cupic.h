@@ -162,6 +162,10 @@ int cupic_set_particles(cupic_run *r, int species, const float *x, const float *
                         const float *vx, const float *vy, int64_t n);
 /* cell offsets of the sorted state, ncx*ncy + 1 entries */
 int cupic_get_offsets(cupic_run *r, int species, int32_t *offsets);
+/* particles per cell row y in [j ds, (j+1) ds), j = 0..ncy-1: the density
+ * profile n_s(y) of one species (the reference keeps only the net rho;
+ * src/diagnostic.cu:137-215 writes row bookmarks) */
+int cupic_density_profile(cupic_run *r, int species, int64_t *per_row);
 
 #define CUPIC_MESH_RHO 0     /* float, charge density                      */
 #define CUPIC_MESH_PHI 1     /* double, potential                          */

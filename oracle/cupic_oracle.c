@@ -971,3 +971,63 @@ void orc_inject_f32(const orc_params *p, int species, uint64_t first_index, int 
     }
     free(Ex); free(Ey);
 }
+
+/* ---- FP32 mirror of the device Poisson solve (mode 1) ------------------ */
+
+void orc_mirror_rhs(const orc_params *p, const int64_t *acc, double *b64, float *b32)
+{
+    double bi = 0x1p-32 / (p->ds * p->eps0);
+    int64_t nn = (int64_t)p->ncx * (p->ncy + 1);
+    for (int64_t k = 0; k < nn; k++) {
+        int64_t j = k / p->ncx;
+        double bb = (j == 0 || j == p->ncy) ? 0.0 : (double)acc[k] * bi;
+        b64[k] = bb;
+        b32[k] = (float)bb;
+    }
+}
+
+static double mirror_resid(int ncx, int ncy, const double *b64, const float *phi)
+{
+    double rmax = 0.0;
+    for (int j = 1; j < ncy; j++)
+        for (int i = 0; i < ncx; i++) {
+            int64_t k = (int64_t)j * ncx + i;
+            int iw = i == 0 ? ncx - 1 : i - 1, ie = i == ncx - 1 ? 0 : i + 1;
+            double sum = (((b64[k] + (double)phi[(int64_t)j * ncx + iw]) +
+                           (double)phi[(int64_t)j * ncx + ie]) + (double)phi[k + ncx]) +
+                         (double)phi[k - ncx];
+            double r = fabs(0.25 * (sum - 4.0 * (double)phi[k]));
+            if (r > rmax) rmax = r;
+        }
+    return rmax;
+}
+
+int orc_mirror_solve_f32(int ncx, int ncy, const double *b64, const float *b32, float *phi,
+                         double tol, int max_sweeps, int check_every, double *resid_out)
+{
+    int64_t nn = (int64_t)ncx * (ncy + 1);
+    float *tmp = (float *)malloc((size_t)nn * sizeof(float));
+    memcpy(tmp, phi, (size_t)nn * sizeof(float));
+    if (check_every < 1) check_every = 1;
+    int s = 0;
+    double res = mirror_resid(ncx, ncy, b64, phi);
+    while (!(res <= tol) && s < max_sweeps) {
+        int k = check_every < max_sweeps - s ? check_every : max_sweeps - s;
+        for (int q = 0; q < k; q++) {
+            for (int j = 1; j < ncy; j++)
+                for (int i = 0; i < ncx; i++) {
+                    int64_t c = (int64_t)j * ncx + i;
+                    int iw = i == 0 ? ncx - 1 : i - 1, ie = i == ncx - 1 ? 0 : i + 1;
+                    float w = phi[(int64_t)j * ncx + iw], e = phi[(int64_t)j * ncx + ie];
+                    tmp[c] = 0.25f * ((((b32[c] + w) + e) + phi[c + ncx]) + phi[c - ncx]);
+                }
+            for (int j = 1; j < ncy; j++)
+                memcpy(phi + (int64_t)j * ncx, tmp + (int64_t)j * ncx, (size_t)ncx * sizeof(float));
+        }
+        s += k;
+        res = mirror_resid(ncx, ncy, b64, phi);
+    }
+    free(tmp);
+    if (resid_out) *resid_out = res;
+    return s;
+}

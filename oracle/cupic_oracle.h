@@ -145,6 +145,18 @@ int64_t orc_mirror_step(const orc_f32c *c, int species, int64_t n, const float *
 void orc_inject_f32(const orc_params *p, int species, uint64_t first_index, int n_in,
                     double t, const double *tin_seq, const float *Exy, float *x, float *y,
                     float *vx, float *vy);
+/* Right-hand side of the device solve from the int64 node charge:
+ * b64 = acc * 2^-32 / (ds eps0) on rows 1..ncy-1, 0 on the Dirichlet rows;
+ * b32 = (float)b64. */
+void orc_mirror_rhs(const orc_params *p, const int64_t *acc, double *b64, float *b32);
+/* The device's FP32 Jacobi with FP64 residual (mode 1): FP32 sweeps
+ * 0.25f * ((((b + west) + east) + north) + south); FP64 residual
+ * max |0.25 ((((b64 + w) + e) + n) + s - 4 phi)| checked before the first
+ * sweep and after every check_every sweeps; stops at the first check with
+ * residual <= tol, else after max_sweeps.  phi [ncy+1][ncx] in/out;
+ * returns the sweeps run and the last residual. */
+int orc_mirror_solve_f32(int ncx, int ncy, const double *b64, const float *b32, float *phi,
+                         double tol, int max_sweeps, int check_every, double *resid_out);
 
 #ifdef __cplusplus
 }

@@ -109,6 +109,9 @@ def _load(omp=False):
                                         C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
         "orc_inject_f32": (None, [P, C.c_int, C.c_uint64, C.c_int, C.c_double, f64p, f32p,
                                   f32p, f32p, f32p, f32p]),
+        "orc_mirror_rhs": (None, [P, i64p, f64p, f32p]),
+        "orc_mirror_solve_f32": (C.c_int, [C.c_int, C.c_int, f64p, f32p, f32p, C.c_double,
+                                           C.c_int, C.c_int, C.POINTER(C.c_double)]),
     }
     for k, (res, args) in sig.items():
         fn = getattr(lib, k)

@@ -91,3 +91,69 @@ def rel_err(a, b):
     """Norm-wise relative error max|a-b| / max|b|."""
     b = np.asarray(b, np.float64)
     return float(np.abs(np.asarray(a, np.float64) - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+class MirrorRun:
+    """A whole run replayed by the oracle's FP32 mirror, independently of the
+    GPU: initial load and sort, fixed-point deposit, the mode-1 FP32 Jacobi
+    with FP64 residual, the field, the half-step stagger, then per step the
+    cycle deposit -> solve -> field -> push/absorb/inject/sort (the order of
+    init_sim and of the time loop, src/init.cu:57-102, src/main.cu:61-73)."""
+
+    def __init__(self, p):
+        self.op = op = oracle_params(p)
+        self.c = O.f32c(op)
+        self.tol, self.tol_init = p.jacobi_tol, p.jacobi_tol_init
+        self.max, self.check = p.jacobi_max_sweeps, p.jacobi_check_every
+        self.nn = op.ncx * (op.ncy + 1)
+        phi = np.zeros(self.nn)
+        O.lib().orc_phi_linear(op.ncx, op.ncy, op.phi_p, phi)
+        self.phi = phi.astype(np.float32)
+        self.st = [list(mirror_init(op, s)[0]) for s in (0, 1)]
+        self.off = [None, None]
+        self.tin, self.ninj, self.started = [0.0, 0.0], [0, 0], False
+        self.total_probe = [0, 0]
+        self.sweeps, self.resid = self.solve(self.tol_init)
+        self.E = self.field()
+        for s in (0, 1):
+            x, y, vx, vy = self.st[s]
+            O.lib().orc_mirror_half_kick(self.c, s, len(x), x, y, vx, vy, self.E)
+
+    def solve(self, tol):
+        acc = fixed_charge(self.op, self.st).ravel()
+        b64 = np.empty(self.nn)
+        b32 = np.empty(self.nn, np.float32)
+        O.lib().orc_mirror_rhs(self.op, acc, b64, b32)
+        res = C.c_double()
+        sw = O.lib().orc_mirror_solve_f32(self.op.ncx, self.op.ncy, b64, b32, self.phi, tol,
+                                          self.max, self.check, C.byref(res))
+        return sw, res.value
+
+    def field(self):
+        ex = np.empty(self.nn)
+        ey = np.empty(self.nn)
+        O.lib().orc_field(self.op.ncx, self.op.ncy, self.op.ds, self.phi.astype(np.float64),
+                          ex, ey)
+        E = np.empty(2 * self.nn, np.float32)
+        E[0::2] = ex.astype(np.float32)
+        E[1::2] = ey.astype(np.float32)
+        return E
+
+    def step(self, t):
+        self.sweeps, self.resid = self.solve(self.tol)
+        self.E = self.field()
+        for s in (0, 1):
+            inj, tin_after = injection_for_step(self.op, s, t, self.tin[s], self.started,
+                                                self.ninj[s], self.E)
+            out, off, pr, tp, nf = mirror_step(self.op, s, self.st[s], self.E, inj)
+            self.st[s] = out
+            self.off[s] = off
+            self.total_probe[s] += pr
+            self.ninj[s] += len(inj[0])
+            self.tin[s] = tin_after
+        self.started = True
+
+    def row_profile(self, s):
+        """particles per cell row"""
+        rows = np.floor(self.st[s][1].astype(np.float64) / self.op.ds).astype(np.int64)
+        return np.bincount(np.minimum(rows, self.op.ncy - 1), minlength=self.op.ncy)

@@ -256,3 +256,70 @@ def test_field_sinusoid_closed_form():
     assert np.allclose(Ex[1:-1], want[1:-1], rtol=0, atol=1e-13)
     assert np.all(Ex[0] == 0) and np.all(Ex[-1] == 0)
     assert np.abs(Ey).max() < 1e-13
+
+
+# ---------------------------------------------------------------- FP32 mirror solve
+
+def mirror_case(ncx=24, ncy=12, n=3000):
+    import ctypes as C
+    p = small(ncx=ncx, ncy=ncy)
+    x, y = random_positions(p, n, seed=5)
+    acc = depfix(p, x, y, 1).ravel()          # ions only: a net charge
+    b64 = np.empty(O.nodes(p))
+    b32 = np.empty(O.nodes(p), np.float32)
+    L.orc_mirror_rhs(p, acc, b64, b32)
+    phi = np.zeros(O.nodes(p))
+    L.orc_phi_linear(p.ncx, p.ncy, p.phi_p, phi)
+    return C, p, acc, b64, b32, phi
+
+
+def numpy_residual(p, b64, phi32):
+    ph = phi32.astype(np.float64).reshape(p.ncy + 1, p.ncx)
+    b = b64.reshape(p.ncy + 1, p.ncx)
+    s = (((b + np.roll(ph, 1, 1)) + np.roll(ph, -1, 1)) + np.roll(ph, -1, 0)) + np.roll(ph, 1, 0)
+    return float(np.abs(0.25 * (s - 4.0 * ph))[1:p.ncy].max())
+
+
+def test_mirror_rhs_is_charge_over_ds_eps0():
+    C, p, acc, b64, b32, _ = mirror_case()
+    want = acc.astype(np.float64) * 2.0 ** -32 / (p.ds * p.n)
+    want = want.reshape(p.ncy + 1, p.ncx)
+    want[0] = want[-1] = 0.0
+    assert np.allclose(b64.reshape(want.shape), want, rtol=1e-15, atol=0)
+    assert np.array_equal(b32, b64.astype(np.float32))
+
+
+def test_mirror_solve_matches_fp64_sweeps():
+    """The FP32 sweeps track the FP64 Jacobi iterates within FP32 rounding."""
+    C, p, acc, b64, b32, phi = mirror_case()
+    phi32 = phi.astype(np.float32)
+    res = C.c_double()
+    sw = L.orc_mirror_solve_f32(p.ncx, p.ncy, b64, b32, phi32, 0.0, 200, 50, C.byref(res))
+    assert sw == 200
+    rho = np.zeros(O.nodes(p))
+    L.orc_rho_from_Q(p.ncx, p.ncy, p.ds, acc.astype(np.float64) * 2.0 ** -32, rho)
+    work = np.zeros_like(phi)
+    last = C.c_double()
+    L.orc_jacobi_sweeps(p.ncx, p.ncy, p.ds, p.eps0, rho, phi, work, 200, C.byref(last))
+    assert np.abs(phi32 - phi).max() / np.abs(phi).max() < 2e-6
+    assert res.value == numpy_residual(p, b64, phi32)
+
+
+def test_mirror_solve_stopping_rule():
+    C, p, acc, b64, b32, phi = mirror_case()
+    r0 = numpy_residual(p, b64, phi.astype(np.float32))
+    res = C.c_double()
+    phi32 = phi.astype(np.float32)
+    assert L.orc_mirror_solve_f32(p.ncx, p.ncy, b64, b32, phi32, r0, 500, 50, C.byref(res)) == 0
+    assert res.value == r0
+    # a tolerance reached mid-way stops at the first check point below it
+    phi32 = phi.astype(np.float32)
+    full = []
+    for k in range(1, 7):
+        q = phi.astype(np.float32)
+        L.orc_mirror_solve_f32(p.ncx, p.ncy, b64, b32, q, 0.0, 50 * k, 50, C.byref(res))
+        full.append(res.value)
+    tol = 0.5 * (full[2] + full[3])        # between the checks at 150 and 200 sweeps
+    assert full[3] < tol < full[2]
+    sw = L.orc_mirror_solve_f32(p.ncx, p.ncy, b64, b32, phi32, tol, 500, 50, C.byref(res))
+    assert sw == 200 and res.value == full[3]
