@@ -169,6 +169,7 @@ __device__ __forceinline__ void st4(float4 *p, int i, v4f v)
     ((CUPIC_GLOBAL v4f *)p)[i] = v;
 }
 
+
 // float2 is a class type that cannot be copied through an address-space
 // reference; load it as a native two-element vector.
 __device__ __forceinline__ float2 ld_f2(const float2 *p, int k)

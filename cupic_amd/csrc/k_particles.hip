@@ -442,8 +442,8 @@ __device__ __forceinline__ Pushed push_tile(const Geo &R, const float2 *Et, int 
 // position and deposits its charge.
 template <bool kScatter>
 __device__ __forceinline__ void push_task(const RunDev &R, RunState *st, const TaskInfo &T,
-                                          int *cntw, float2 *Et, unsigned long long *tile,
-                                          int lane)
+                                          int *cntw, int *gonew, float2 *Et,
+                                          unsigned long long *tile, int lane)
 {
     const int s = T.s, j0 = T.j0, col0 = T.col0, nc = T.nc;
     const Geo G = geo_of(R);
@@ -481,6 +481,7 @@ __device__ __forceinline__ void push_task(const RunDev &R, RunState *st, const T
         for (int q = lane; q < 4 * TW; q += kWave) tile[q] = 0ull;
     } else {
         for (int q = lane; q < nc * 9; q += kWave) cntw[q] = 0;
+        for (int q = lane; q < nc; q += kWave) gonew[q] = 0;
     }
     wave_lds_sync();
     const unsigned long long lt = (1ull << lane) - 1ull;
@@ -506,15 +507,26 @@ __device__ __forceinline__ void push_task(const RunDev &R, RunState *st, const T
             dir = kDirFar;
             bad = true;
         }
+        int rank = 0;
+        int seg_start = lane;
+        bool one_cell = false;
+        int cl0 = 0;
+        unsigned long long heads = 0ull;
+        if (!kScatter) {
+            // pass 1 needs counts only: movers and departures are counted per
+            // (old cell, direction) with LDS atomics (few lanes); stayers are
+            // the cell's population minus those, once per cell after the loop
+            if (valid && dir != 4 && cl >= 0 && cl < nc) {
+                if (dir < 9) atomicAdd(&cntw[cl * 9 + dir], 1);
+                else atomicAdd(&gonew[cl], 1);
+            }
+        } else {
         // rank within (old cell, direction); lanes of one old cell are contiguous
         int clp = dpp_prev_int(cl);
         bool head = valid && (lane == 0 || cl != clp);
-        unsigned long long heads = __ballot(head);
+        heads = __ballot(head);
         unsigned long long near_m = __ballot(dir < 9);
-        int rank = 0;
-        int seg_start = lane;
-        const bool one_cell = (heads == 1ull);
-        int cl0 = 0;
+        one_cell = (heads == 1ull);
         if (one_cell) {
             // the whole chunk lies in one old cell: per direction present,
             // rank = counter + earlier lanes with that direction
@@ -548,6 +560,7 @@ __device__ __forceinline__ void push_task(const RunDev &R, RunState *st, const T
             }
             wave_lds_sync();
             if (dir < 9 && last) cntw[cl * 9 + dir] = rank + 1;
+        }
         }
         wave_lds_sync();
         if (!kScatter) {
@@ -630,6 +643,15 @@ __device__ __forceinline__ void push_task(const RunDev &R, RunState *st, const T
     if (kScatter && acc_cl >= 0) flush_stayers(tile, TW, acc_cl, acc1, acc2, sgn, lane);
     wave_lds_sync();
     if (!kScatter) {
+        // stayers = population - movers - departures (probe, top, far)
+        for (int cl = lane; cl < nc; cl += kWave) {
+            int moved = gonew[cl];
+#pragma unroll
+            for (int d = 0; d < 9; d++)
+                if (d != 4) moved += cntw[cl * 9 + d];
+            cntw[cl * 9 + 4] = off[c0 + cl + 1] - off[c0 + cl] - moved;
+        }
+        wave_lds_sync();
         CUPIC_GLOBAL int *cnt = gp(R.cnt[s]);
         for (int q = lane; q < nc * 9; q += kWave) {
             int cl = q / 9, d = q - 9 * cl;
@@ -651,11 +673,12 @@ __global__ __launch_bounds__(kBlock) void k_push_rank(const RunDev *rds, const i
 {
     RUN_PROLOGUE();
     __shared__ int cnts[kBlock / kWave][kCmax * 9];
+    __shared__ int gone[kBlock / kWave][kCmax];
     __shared__ float2 etiles[kBlock / kWave][2 * (kCmax + 1)];
     TaskInfo T = task_of(R, local);
     if (!T.ok) return;
     int w = threadIdx.x >> 6;
-    push_task<false>(R, st, T, cnts[w], etiles[w], nullptr, threadIdx.x & (kWave - 1));
+    push_task<false>(R, st, T, cnts[w], gone[w], etiles[w], nullptr, threadIdx.x & (kWave - 1));
 }
 
 __global__ __launch_bounds__(kBlock) void k_scatter(const RunDev *rds, const int *start,
@@ -668,7 +691,7 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const RunDev *rds, const int
     TaskInfo T = task_of(R, local);
     if (!T.ok) return;
     int w = threadIdx.x >> 6;
-    push_task<true>(R, st, T, cnts[w], etiles[w], tiles[w], threadIdx.x & (kWave - 1));
+    push_task<true>(R, st, T, cnts[w], nullptr, etiles[w], tiles[w], threadIdx.x & (kWave - 1));
 }
 
 // ---------------------------------------------------------------- launchers
