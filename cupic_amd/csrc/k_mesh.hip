@@ -170,17 +170,18 @@ __global__ __launch_bounds__(kBlock) void k_jacobi_sweep(const RunDev *rds, cons
 
 // Neighbour values one lane away, as VALU data-parallel-primitive moves:
 // wave_shr:1 gives lane l the value of lane l-1, wave_shl:1 that of lane
-// l+1 (lanes 0 and 63 keep their own value; they are never updated).
+// l+1.  Lanes 0 and 63 read 0 (they are never updated nor stored); with
+// zero fill the compiler folds each move into the add that consumes it.
 __device__ __forceinline__ float lane_prev(float v)
 {
-    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v),
-                                                      0x138, 0xF, 0xF, false));
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xF, 0xF,
+                                                      true));
 }
 
 __device__ __forceinline__ float lane_next(float v)
 {
-    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v),
-                                                      0x130, 0xF, 0xF, false));
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xF, 0xF,
+                                                      true));
 }
 
 // nsw <= kJacH - 1 Jacobi sweeps s0 .. s0+nsw-1 in one launch (modes 1, 2).
