@@ -31,10 +31,12 @@ constexpr int kScanCells = 1024;       // cells per scan block
 constexpr int kRedSlots = 16;         // atomic slots per reduced quantity
 constexpr int kGT = 64;        // direct solve: GEMM output tile
 constexpr int kGK = 16;        // direct solve: GEMM k tile
-constexpr int kJacRows = 16;           // blocked Jacobi: rows per wave
+constexpr int kJacRows = 8;            // blocked Jacobi: rows per wave
+constexpr int kJacWaves = 8;           // blocked Jacobi: waves per workgroup
+constexpr int kJacThreads = kJacWaves * 64;
 constexpr int kJacH = 11;              // halo; at most kJacH - 1 sweeps per launch
 constexpr int kJacTX = 64 - 2 * kJacH;                   // tile columns (42)
-constexpr int kJacTY = kJacRows * (256 / 64) - 2 * kJacH;  // tile rows (42)
+constexpr int kJacTY = kJacRows * kJacWaves - 2 * kJacH; // tile rows
 
 enum Dir : int { kDirFar = 9, kDirProbe = 10, kDirTop = 11, kDirNone = 15 };
 

@@ -108,7 +108,7 @@ __device__ __forceinline__ void mark_done(const RunDev &R, RunState *st, int loc
 }
 
 // block-wide max of one value per thread, into a slotted atomic
-template <class T, class F>
+template <int NW = kBlock / kWave, class T, class F>
 __device__ __forceinline__ void block_max_atomic(T v, T *sh, F emit)
 {
     int lane = threadIdx.x & (kWave - 1), w = threadIdx.x >> 6;
@@ -121,7 +121,7 @@ __device__ __forceinline__ void block_max_atomic(T v, T *sh, F emit)
     __syncthreads();
     if (threadIdx.x == 0) {
         T m = sh[0];
-        for (int q = 1; q < kBlock / kWave; q++) m = m > sh[q] ? m : sh[q];
+        for (int q = 1; q < NW; q++) m = m > sh[q] ? m : sh[q];
         emit(m);
     }
 }
@@ -185,7 +185,7 @@ __device__ __forceinline__ float lane_next(float v)
 }
 
 // nsw <= kJacH - 1 Jacobi sweeps s0 .. s0+nsw-1 in one launch (modes 1, 2).
-// A workgroup holds a 64 x (4 * kJacRows) node region -- a kJacTX x kJacTY
+// A workgroup holds a 64 x (kJacWaves * kJacRows) node region -- a kJacTX x kJacTY
 // tile with a kJacH-node halo -- in registers: lane = region column, each
 // wave kJacRows consecutive rows.  East/west neighbours come from the next
 // lanes, north/south from the thread's own rows; only the rows at wave
@@ -196,15 +196,15 @@ __device__ __forceinline__ float lane_next(float v)
 // d starting at 0 when first.  par selects the input buffer; the output goes
 // to the other one.  With resid_slot >= 0 (mode 1) the launch also reduces
 // the FP64 residual of its tile after the last sweep.
-__global__ __launch_bounds__(kBlock) void k_jacobi_block(const RunDev *rds, const int *start,
+__global__ __launch_bounds__(kJacThreads) void k_jacobi_block(const RunDev *rds, const int *start,
                                                          int nruns, int s0, int nsw, int mode,
                                                          int check_slot, int first, int par,
                                                          int resid_slot)
 {
     RUN_PROLOGUE();
     constexpr int NR = kJacRows;
-    __shared__ float xch[2][kBlock / kWave][2][kWave];
-    __shared__ double shr[kBlock / kWave];
+    __shared__ float xch[2][kJacWaves][2][kWave];
+    __shared__ double shr[kJacWaves];
     if (st->solve_done) return;
     if (check_slot >= 0) {
         double res = read_resid(R, check_slot);
@@ -246,7 +246,7 @@ __global__ __launch_bounds__(kBlock) void k_jacobi_block(const RunDev *rds, cons
         bool ok = gy >= 0 && gy <= ncy;
         v[i] = (ok && !zero_in) ? vin : 0.0f;
         b[i] = ok ? bin : 0.0f;
-        if (lane_in && lr >= 1 && lr <= NR * (kBlock / kWave) - 2 && gy >= 1 && gy < ncy)
+        if (lane_in && lr >= 1 && lr <= NR * kJacWaves - 2 && gy >= 1 && gy < ncy)
             upd |= 1u << i;
     }
 #pragma unroll 1
@@ -256,7 +256,7 @@ __global__ __launch_bounds__(kBlock) void k_jacobi_block(const RunDev *rds, cons
         x[w][1][lane] = v[NR - 1];
         __syncthreads();
         float prev = w > 0 ? x[w - 1][1][lane] : 0.0f;  // old value of the row below
-        const float above = w < kBlock / kWave - 1 ? x[w + 1][0][lane] : 0.0f;
+        const float above = w < kJacWaves - 1 ? x[w + 1][0][lane] : 0.0f;
 #pragma unroll
         for (int i = 0; i < NR; i++) {
             const float cur = v[i];
@@ -282,7 +282,7 @@ __global__ __launch_bounds__(kBlock) void k_jacobi_block(const RunDev *rds, cons
     x[w][1][lane] = v[NR - 1];
     __syncthreads();
     const float below = w > 0 ? x[w - 1][1][lane] : 0.0f;
-    const float above = w < kBlock / kWave - 1 ? x[w + 1][0][lane] : 0.0f;
+    const float above = w < kJacWaves - 1 ? x[w + 1][0][lane] : 0.0f;
     const CUPIC_GLOBAL double *b64 = gp(R.b64);
     double rmax = 0.0;
 #pragma unroll
@@ -299,7 +299,7 @@ __global__ __launch_bounds__(kBlock) void k_jacobi_block(const RunDev *rds, cons
         }
     }
     unsigned long long *slot = &R.resid[resid_slot * kRedSlots + (local & (kRedSlots - 1))];
-    block_max_atomic(rmax, shr, [&](double m) {
+    block_max_atomic<kJacWaves>(rmax, shr, [&](double m) {
         if (m > 0.0) atomicMax(slot, (unsigned long long)__double_as_longlong(m));
     });
 }
@@ -429,7 +429,9 @@ void launch_jacobi_sweep(const Launch &L, int s) { LAUNCH_K(k_jacobi_sweep, L, s
 void launch_jacobi_block(const Launch &L, int s0, int nsw, int mode, int check_slot, int first,
                          int par, int resid_slot)
 {
-    LAUNCH_K(k_jacobi_block, L, s0, nsw, mode, check_slot, first, par, resid_slot);
+    if (L.nblocks > 0)
+        hipLaunchKernelGGL(k_jacobi_block, dim3(L.nblocks), dim3(kJacThreads), 0, L.stream, L.rds,
+                           L.start, L.nruns, s0, nsw, mode, check_slot, first, par, resid_slot);
 }
 void launch_ir_update(const Launch &L, int par) { LAUNCH_K(k_ir_update, L, par); }
 void launch_residual(const Launch &L, int c, int mode, int par)
