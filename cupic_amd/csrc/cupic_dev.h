@@ -299,6 +299,18 @@ __device__ __forceinline__ void cell_of(const G &R, float x, float y, float &fx,
 }
 
 // CIC gather from interleaved (Ex, Ey), fixed operation order.
+// CIC interpolation of (Ex, Ey) from the four corners of a cell, weights
+// (1-fx)(1-fy), fx(1-fy), (1-fx)fy, fx fy accumulated in that order with
+// fused multiply-adds.
+__device__ __forceinline__ void cic_interp(float fx, float fy, float2 e00, float2 e10,
+                                           float2 e01, float2 e11, float &ex, float &ey)
+{
+    float gx = 1.0f - fx, gy = 1.0f - fy;
+    float w00 = gx * gy, w10 = fx * gy, w01 = gx * fy, w11 = fx * fy;
+    ex = fmaf(w11, e11.x, fmaf(w01, e01.x, fmaf(w10, e10.x, w00 * e00.x)));
+    ey = fmaf(w11, e11.y, fmaf(w01, e01.y, fmaf(w10, e10.y, w00 * e00.y)));
+}
+
 template <class G>
 __device__ __forceinline__ void gather(const G &R, const float2 *__restrict__ Eg, float fx,
                                        float fy, int ic, int jc, float &ex, float &ey)
@@ -307,18 +319,18 @@ __device__ __forceinline__ void gather(const G &R, const float2 *__restrict__ Eg
     int k00 = jc * R.ncx + ic, k10 = jc * R.ncx + i1;
     int k01 = k00 + R.ncx, k11 = k10 + R.ncx;
     float2 e00 = ld_f2(Eg, k00), e10 = ld_f2(Eg, k10), e01 = ld_f2(Eg, k01), e11 = ld_f2(Eg, k11);
-    float gx = 1.0f - fx, gy = 1.0f - fy;
-    float w00 = gx * gy, w10 = fx * gy, w01 = gx * fy, w11 = fx * fy;
-    float a = w00 * e00.x;
-    a = a + w10 * e10.x;
-    a = a + w01 * e01.x;
-    a = a + w11 * e11.x;
-    float b = w00 * e00.y;
-    b = b + w10 * e10.y;
-    b = b + w01 * e01.y;
-    b = b + w11 * e11.y;
-    ex = a;
-    ey = b;
+    cic_interp(fx, fy, e00, e10, e01, e11, ex, ey);
+}
+
+// The leapfrog update v += (q dt / m) E, x += dt v, each as one fused
+// multiply-add.
+__device__ __forceinline__ void leapfrog(float qdtm, float dt, float ex, float ey, float &x,
+                                         float &y, float &vx, float &vy)
+{
+    vx = fmaf(qdtm, ex, vx);
+    vy = fmaf(qdtm, ey, vy);
+    x = fmaf(dt, vx, x);
+    y = fmaf(dt, vy, y);
 }
 
 struct Pushed {
@@ -343,10 +355,8 @@ __device__ __forceinline__ Pushed push_one(const G &R, const float2 *__restrict_
     cell_of(R, x, y, fx, fy, oic, ojc);
     float ex, ey;
     gather(R, E, fx, fy, oic, ojc, ex, ey);
-    float nvx = vx + qdtm * ex;
-    float nvy = vy + qdtm * ey;
-    float nx = x + R.dt_f * nvx;
-    float ny = y + R.dt_f * nvy;
+    float nx = x, ny = y, nvx = vx, nvy = vy;
+    leapfrog(qdtm, R.dt_f, ex, ey, nx, ny, nvx, nvy);
     if (nx < 0.0f) nx = nx + R.Lx_f;
     if (nx >= R.Lx_f) nx = nx - R.Lx_f;
     o.x = nx; o.y = ny; o.vx = nvx; o.vy = nvy;

@@ -285,8 +285,8 @@ __global__ __launch_bounds__(kBlock) void k_half_kick(const RunDev *rds, const i
         v4f p = ld4(P, i);
         cell_of(R, p.x, p.y, fx, fy, ic, jc);
         gather(R, R.E, fx, fy, ic, jc, ex, ey);
-        p.z = p.z + hk * ex;
-        p.w = p.w + hk * ey;
+        p.z = fmaf(hk, ex, p.z);
+        p.w = fmaf(hk, ey, p.w);
         st4(P, i, p);
     }
 }
@@ -397,22 +397,12 @@ __device__ __forceinline__ Pushed push_tile(const Geo &R, const float2 *Et, int 
     cl = min(max(cl, 0), nc - 1);
     int w = nc + 1;
     float2 e00 = Et[cl], e10 = Et[cl + 1], e01 = Et[w + cl], e11 = Et[w + cl + 1];
-    float gx = 1.0f - fx, gy = 1.0f - fy;
-    float w00 = gx * gy, w10 = fx * gy, w01 = gx * fy, w11 = fx * fy;
-    float ex = w00 * e00.x;
-    ex = ex + w10 * e10.x;
-    ex = ex + w01 * e01.x;
-    ex = ex + w11 * e11.x;
-    float ey = w00 * e00.y;
-    ey = ey + w10 * e10.y;
-    ey = ey + w01 * e01.y;
-    ey = ey + w11 * e11.y;
+    float ex, ey;
+    cic_interp(fx, fy, e00, e10, e01, e11, ex, ey);
     (void)E;
     Pushed o;
-    float nvx = vx + qdtm * ex;
-    float nvy = vy + qdtm * ey;
-    float nx = x + R.dt_f * nvx;
-    float ny = y + R.dt_f * nvy;
+    float nx = x, ny = y, nvx = vx, nvy = vy;
+    leapfrog(qdtm, R.dt_f, ex, ey, nx, ny, nvx, nvy);
     if (nx < 0.0f) nx = nx + R.Lx_f;
     if (nx >= R.Lx_f) nx = nx - R.Lx_f;
     o.x = nx; o.y = ny; o.vx = nvx; o.vy = nvy;

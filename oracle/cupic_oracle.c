@@ -799,7 +799,9 @@ void orc_deposit_fixed_f32(int64_t n, const float *x, const float *y, const orc_
     }
 }
 
-/* FP32 CIC gather from interleaved (Ex, Ey) nodes, fixed operation order. */
+/* FP32 CIC gather from interleaved (Ex, Ey) nodes, fixed operation order:
+ * the weighted corners accumulated with fused multiply-adds, as the device
+ * does. */
 void orc_gather_f32(const orc_f32c *c, const float *Exy, float x, float y, float *ex,
                     float *ey)
 {
@@ -812,16 +814,10 @@ void orc_gather_f32(const orc_f32c *c, const float *Exy, float x, float y, float
     int64_t k01 = k00 + ncx, k11 = k10 + ncx;
     float gx = 1.0f - fx, gy = 1.0f - fy;
     float w00 = gx * gy, w10 = fx * gy, w01 = gx * fy, w11 = fx * fy;
-    float a = w00 * Exy[2 * k00];
-    a = a + w10 * Exy[2 * k10];
-    a = a + w01 * Exy[2 * k01];
-    a = a + w11 * Exy[2 * k11];
-    float b = w00 * Exy[2 * k00 + 1];
-    b = b + w10 * Exy[2 * k10 + 1];
-    b = b + w01 * Exy[2 * k01 + 1];
-    b = b + w11 * Exy[2 * k11 + 1];
-    *ex = a;
-    *ey = b;
+    *ex = fmaf(w11, Exy[2 * k11],
+               fmaf(w01, Exy[2 * k01], fmaf(w10, Exy[2 * k10], w00 * Exy[2 * k00])));
+    *ey = fmaf(w11, Exy[2 * k11 + 1],
+               fmaf(w01, Exy[2 * k01 + 1], fmaf(w10, Exy[2 * k10 + 1], w00 * Exy[2 * k00 + 1])));
 }
 
 int64_t orc_stable_sort_slots(int64_t n, const int32_t *key, const int8_t *slot,
@@ -878,8 +874,8 @@ void orc_mirror_half_kick(const orc_f32c *c, int species, int64_t n, const float
     for (int64_t k = 0; k < n; k++) {
         float ex, ey;
         orc_gather_f32(c, Exy, x[k], y[k], &ex, &ey);
-        vx[k] = vx[k] + hk * ex;
-        vy[k] = vy[k] + hk * ey;
+        vx[k] = fmaf(hk, ex, vx[k]);
+        vy[k] = fmaf(hk, ey, vy[k]);
     }
 }
 
@@ -904,10 +900,11 @@ int64_t orc_mirror_step(const orc_f32c *c, int species, int64_t n, const float *
         orc_cell_of_f32(c, x[k], y[k], &fx, &fy, &oic, &ojc);
         float ex, ey;
         orc_gather_f32(c, Exy, x[k], y[k], &ex, &ey);
-        float nvx = vx[k] + qdtm * ex;
-        float nvy = vy[k] + qdtm * ey;
-        float nx = x[k] + c->dt * nvx;
-        float ny = y[k] + c->dt * nvy;
+        /* v += (q dt / m) E, x += dt v as fused multiply-adds (device order) */
+        float nvx = fmaf(qdtm, ex, vx[k]);
+        float nvy = fmaf(qdtm, ey, vy[k]);
+        float nx = fmaf(c->dt, nvx, x[k]);
+        float ny = fmaf(c->dt, nvy, y[k]);
         if (nx < 0.0f) nx = nx + c->Lx;
         if (nx >= c->Lx) nx = nx - c->Lx;
         px[k] = nx; py[k] = ny; pvx[k] = nvx; pvy[k] = nvy;
