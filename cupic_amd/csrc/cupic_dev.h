@@ -34,8 +34,8 @@ constexpr int kGK = 16;        // direct solve: GEMM k tile
 constexpr int kJacRows = 8;            // blocked Jacobi: rows per wave
 constexpr int kJacWaves = 8;           // blocked Jacobi: waves per workgroup
 constexpr int kJacThreads = kJacWaves * 64;
-constexpr int kJacH = 11;              // halo; at most kJacH - 1 sweeps per launch
-constexpr int kJacTX = 64 - 2 * kJacH;                   // tile columns (42)
+constexpr int kJacH = 16;              // halo; at most kJacH - 1 sweeps per launch
+constexpr int kJacTX = 64 - 2 * kJacH;                   // tile columns
 constexpr int kJacTY = kJacRows * kJacWaves - 2 * kJacH; // tile rows
 
 enum Dir : int { kDirFar = 9, kDirProbe = 10, kDirTop = 11, kDirNone = 15 };
