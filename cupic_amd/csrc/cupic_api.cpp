@@ -114,13 +114,13 @@ int shape_blocks(const RunDev &h, Shape s)
         return (int)(((int64_t)h.ncx * 2 * h.nk + kBlock - 1) / kBlock) + (h.nk + kBlock - 1) / kBlock;
     case kShapeDftFwd:
         if (!h.dft_f) return 0;
-        return ((h.ncy + 1 + kGT - 1) / kGT) * ((2 * h.nk + kGT - 1) / kGT);
+        return ((h.ncy + 1 + kGM - 1) / kGM) * ((2 * h.nk + kGN - 1) / kGN);
     case kShapeDftTri:
         if (!h.dft_f) return 0;
         return (2 * h.nk + kBlock - 1) / kBlock;
     case kShapeDftInv:
         if (!h.dft_f) return 0;
-        return ((h.ncy - 1 + kGT - 1) / kGT) * ((h.ncx + kGT - 1) / kGT);
+        return ((h.ncy - 1 + kGM - 1) / kGM) * ((h.ncx + kGN - 1) / kGN);
     default: return 0;
     }
 }

@@ -29,7 +29,8 @@ constexpr int kCmax = 16;              // max cells per wave task
 constexpr int kTileW = kCmax + 3;      // deposit tile width (node columns)
 constexpr int kScanCells = 1024;       // cells per scan block
 constexpr int kRedSlots = 16;         // atomic slots per reduced quantity
-constexpr int kGT = 64;        // direct solve: GEMM output tile
+constexpr int kGM = 64;        // direct solve: GEMM output tile rows (4 waves x 16)
+constexpr int kGN = 32;        // direct solve: GEMM output tile columns
 constexpr int kGK = 16;        // direct solve: GEMM k tile
 constexpr int kJacRows = 8;            // blocked Jacobi: rows per wave
 constexpr int kJacWaves = 8;           // blocked Jacobi: waves per workgroup

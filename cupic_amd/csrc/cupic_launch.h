@@ -32,9 +32,9 @@ enum Shape : int {
     kShapeInit,        // 2 * ceil(n_init / 256)
     kShapeJacBlk,      // ceil(ncx / kJacTX) * ceil((ncy - 1) / kJacTY)
     kShapeDftInit,     // ceil(ncx * 2K / 256) + ceil(K / 256)
-    kShapeDftFwd,      // ceil((ncy + 1) / kGT) * ceil(2K / kGT)
+    kShapeDftFwd,      // ceil((ncy + 1) / kGM) * ceil(2K / kGN)
     kShapeDftTri,      // ceil(2K / 256)
-    kShapeDftInv,      // ceil((ncy - 1) / kGT) * ceil(ncx / kGT)
+    kShapeDftInv,      // ceil((ncy - 1) / kGM) * ceil(ncx / kGN)
     kShapeCount
 };
 

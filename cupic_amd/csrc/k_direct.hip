@@ -30,61 +30,63 @@ namespace cupic {
     RunState *st = R.st;                                            \
     (void)local; (void)st;
 
-// FP64 GEMM tile: a 256-thread workgroup computes a kGT x kGT block of
-// C = A B (A: M x Kd through load_a, B: Kd x N row-major), 4 x 4 outputs per
-// thread at stride 16, the k dimension staged through LDS kGK at a time.
+// FP64 GEMM tile on the matrix cores: a 256-thread workgroup computes a
+// kGM x kGN block of C = A B (A: M x Kd through load_a, B: Kd x N row-major)
+// with v_mfma_f64_16x16x4f64.  Wave w owns rows 16w..16w+15 of the block
+// and kGN / 16 output blocks of 16 x 16; the k dimension is staged through
+// LDS kGK at a time.  Operand layout of the instruction: lane l supplies
+// A[l % 16][k + l / 16] and B[k + l / 16][l % 16]; it receives
+// D[l / 16 + 4 r][l % 16] in result register r.
+typedef double v4d __attribute__((ext_vector_type(4)));
+
 template <class LoadA, class StoreC>
 __device__ __forceinline__ void gemm_tile(int M, int N, int Kd, int tile, LoadA load_a,
                                           const double *__restrict__ B, StoreC store_c)
 {
-    __shared__ double As[kGK][kGT + 1];
-    __shared__ double Bs[kGK][kGT + 1];
-    const int ntn = (N + kGT - 1) / kGT;
+    constexpr int NB = kGN / 16;
+    __shared__ double As[kGK][kGM + 1];
+    __shared__ double Bs[kGK][kGN + 1];
+    const int ntn = (N + kGN - 1) / kGN;
     const int tm = tile / ntn, tn = tile - tm * ntn;
-    const int row0 = tm * kGT, col0 = tn * kGT;
-    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-    double acc[4][4];
+    const int row0 = tm * kGM, col0 = tn * kGN;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    v4d acc[NB];
 #pragma unroll
-    for (int a = 0; a < 4; a++)
-#pragma unroll
-        for (int b = 0; b < 4; b++) acc[a][b] = 0.0;
+    for (int nb = 0; nb < NB; nb++) acc[nb] = v4d{0.0, 0.0, 0.0, 0.0};
     for (int k0 = 0; k0 < Kd; k0 += kGK) {
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            int e = threadIdx.x + q * kBlock;   // 64 rows x 16 k
-            int r = e >> 4, kk = e & 15;
+        for (int q = 0; q < kGM * kGK / kBlock; q++) {
+            int e = threadIdx.x + q * kBlock;   // kGM rows x kGK k, k fastest
+            int r = e / kGK, kk = e - r * kGK;
             int gr = row0 + r, gk = k0 + kk;
             As[kk][r] = (gr < M && gk < Kd) ? load_a(gr, gk) : 0.0;
         }
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            int e = threadIdx.x + q * kBlock;   // 16 k x 64 columns
-            int kk = e >> 6, c = e & 63;
+        for (int q = 0; q < kGK * kGN / kBlock; q++) {
+            int e = threadIdx.x + q * kBlock;   // kGK k x kGN columns, columns fastest
+            int kk = e / kGN, c = e - kk * kGN;
             int gk = k0 + kk, gc = col0 + c;
             Bs[kk][c] = (gk < Kd && gc < N) ? B[(size_t)gk * N + gc] : 0.0;
         }
         __syncthreads();
 #pragma unroll
-        for (int kk = 0; kk < kGK; kk++) {
-            double a[4], b[4];
+        for (int kk = 0; kk < kGK; kk += 4) {
+            double a = As[kk + lk][16 * w + li];
 #pragma unroll
-            for (int i = 0; i < 4; i++) {
-                a[i] = As[kk][ty + 16 * i];
-                b[i] = Bs[kk][tx + 16 * i];
+            for (int nb = 0; nb < NB; nb++) {
+                double b = Bs[kk + lk][16 * nb + li];
+                acc[nb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[nb], 0, 0, 0);
             }
-#pragma unroll
-            for (int i = 0; i < 4; i++)
-#pragma unroll
-                for (int j = 0; j < 4; j++) acc[i][j] = fma(a[i], b[j], acc[i][j]);
         }
         __syncthreads();
     }
 #pragma unroll
-    for (int i = 0; i < 4; i++)
+    for (int nb = 0; nb < NB; nb++)
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            int r = row0 + ty + 16 * i, c = col0 + tx + 16 * j;
-            if (r < M && c < N) store_c(r, c, acc[i][j]);
+        for (int r = 0; r < 4; r++) {
+            int gr = row0 + 16 * w + lk + 4 * r, gc = col0 + 16 * nb + li;
+            if (gr < M && gc < N) store_c(gr, gc, acc[nb][r]);
         }
 }
 
@@ -143,29 +145,60 @@ __global__ __launch_bounds__(kBlock) void k_dft_fwd(const RunDev *rds, const int
 
 // One tridiagonal system per column of Z_hat (real and imaginary parts of
 // each wavenumber), solved in place on rows 1..ncy-1; the Dirichlet rows
-// enter the first and last equations.
+// enter the first and last equations.  Rows go through registers kTri at a
+// time, so the loads of a block are in flight together and only the
+// recurrence itself is sequential.
 __global__ __launch_bounds__(kBlock) void k_dft_tri(const RunDev *rds, const int *start,
                                                     int nruns)
 {
     RUN_PROLOGUE();
+    constexpr int kTri = 16;
     const int K = R.nk, W = 2 * K, ncy = R.ncy, m = ncy - 1;
     const int c = local * kBlock + threadIdx.x;
     if (c >= W) return;
     const int k = c < K ? c : c - K;
     CUPIC_GLOBAL double *Z = gp(R.dft_z);
     const CUPIC_GLOBAL double *T = gp(R.dft_t);
+    const double z0 = Z[c], zn = Z[(size_t)ncy * W + c];
     double y = 0.0;
-    for (int j = 1; j <= m; j++) {
-        double r = Z[(size_t)j * W + c];
-        if (j == 1) r += Z[c];
-        if (j == m) r += Z[(size_t)ncy * W + c];
-        y = (r + y) * T[(size_t)(j - 1) * K + k];
-        Z[(size_t)j * W + c] = y;
+    for (int j0 = 1; j0 <= m; j0 += kTri) {
+        double r[kTri], g[kTri];
+#pragma unroll
+        for (int q = 0; q < kTri; q++) {
+            int j = j0 + q;
+            r[q] = j <= m ? Z[(size_t)j * W + c] : 0.0;
+            g[q] = j <= m ? T[(size_t)(j - 1) * K + k] : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < kTri; q++) {
+            int j = j0 + q;
+            if (j <= m) {
+                double rr = r[q];
+                if (j == 1) rr += z0;
+                if (j == m) rr += zn;
+                y = (rr + y) * g[q];
+                Z[(size_t)j * W + c] = y;
+            }
+        }
     }
+    // back substitution x(j) = y(j) + g(j) x(j+1), x(m) = y(m), from row m-1 down
     double x = y;
-    for (int j = m - 1; j >= 1; j--) {
-        x = Z[(size_t)j * W + c] + T[(size_t)(j - 1) * K + k] * x;
-        Z[(size_t)j * W + c] = x;
+    for (int j1 = m - 1; j1 >= 1; j1 -= kTri) {
+        double r[kTri], g[kTri];
+#pragma unroll
+        for (int q = 0; q < kTri; q++) {
+            int j = j1 - q;
+            r[q] = j >= 1 ? Z[(size_t)j * W + c] : 0.0;
+            g[q] = j >= 1 ? T[(size_t)(j - 1) * K + k] : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < kTri; q++) {
+            int j = j1 - q;
+            if (j >= 1) {
+                x = r[q] + g[q] * x;
+                Z[(size_t)j * W + c] = x;
+            }
+        }
     }
 }
 
