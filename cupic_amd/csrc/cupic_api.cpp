@@ -108,7 +108,11 @@ int shape_blocks(const RunDev &h, Shape s)
     case kShapeRows: return 2 * h.ncy;
     case kShapeInit: return 2 * ((h.n_init + kBlock - 1) / kBlock);
     case kShapeJacBlk:
-        return ((h.ncx + kJacTX - 1) / kJacTX) * ((h.ncy - 1 + kJacTY - 1) / kJacTY);
+        return ((h.ncx + jac_tx(kJacH) - 1) / jac_tx(kJacH)) *
+               ((h.ncy - 1 + jac_ty(kJacH) - 1) / jac_ty(kJacH));
+    case kShapeJacBlkN:
+        return ((h.ncx + jac_tx(kJacHNarrow) - 1) / jac_tx(kJacHNarrow)) *
+               ((h.ncy - 1 + jac_ty(kJacHNarrow) - 1) / jac_ty(kJacHNarrow));
     case kShapeDftInit:
         if (!h.dft_f) return 0;
         return (int)(((int64_t)h.ncx * 2 * h.nk + kBlock - 1) / kBlock) + (h.nk + kBlock - 1) / kBlock;
@@ -214,9 +218,11 @@ void seq_solve(const Target &T, int use_args, int mode, double tol, int min_swee
         launch_solver_end(T.L(kShapeRun), 0, 0);
         return;
     }
-    Launch B = T.L(kShapeJacBlk);
+    const bool narrow = T.nblk[kShapeJacBlk] >= kJacNarrowMinBlocks;
+    const int halo = narrow ? kJacHNarrow : kJacH;
+    Launch B = T.L(narrow ? kShapeJacBlkN : kShapeJacBlk);
     if (check < 1) check = 1;
-    const int per = kJacH - 1;  // sweeps per blocked launch
+    const int per = halo - 1;  // sweeps per blocked launch
     int c = 0, s = 0, nlaunch = 0, dpar = 0;
     if (mode == 1) {
         // FP64 residuals: the initial one here, the others fused into the
@@ -227,7 +233,7 @@ void seq_solve(const Target &T, int use_args, int mode, double tol, int min_swee
             for (int q = 0; q < k;) {
                 int nsw = std::min(per, k - q);
                 bool last = q + nsw >= k;
-                launch_jacobi_block(B, s + q, nsw, 1, q == 0 ? c : -1, 0, nlaunch & 1,
+                launch_jacobi_block(B, halo, s + q, nsw, 1, q == 0 ? c : -1, 0, nlaunch & 1,
                                     last ? c + 1 : -1);
                 nlaunch++;
                 q += nsw;
@@ -246,7 +252,7 @@ void seq_solve(const Target &T, int use_args, int mode, double tol, int min_swee
         int dl = 0;
         for (int q = 0; q < k;) {
             int nsw = std::min(per, k - q);
-            launch_jacobi_block(B, s + q, nsw, 2, q == 0 ? c : -1, q == 0 ? 1 : 0, dl & 1, -1);
+            launch_jacobi_block(B, halo, s + q, nsw, 2, q == 0 ? c : -1, q == 0 ? 1 : 0, dl & 1, -1);
             dl++;
             q += nsw;
         }

@@ -35,9 +35,15 @@ constexpr int kGK = 16;        // direct solve: GEMM k tile
 constexpr int kJacRows = 8;            // blocked Jacobi: rows per wave
 constexpr int kJacWaves = 8;           // blocked Jacobi: waves per workgroup
 constexpr int kJacThreads = kJacWaves * 64;
-constexpr int kJacH = 16;              // halo; at most kJacH - 1 sweeps per launch
-constexpr int kJacTX = 64 - 2 * kJacH;                   // tile columns
-constexpr int kJacTY = kJacRows * kJacWaves - 2 * kJacH; // tile rows
+// Halo of the blocked Jacobi, at most halo - 1 sweeps per launch.  A wide
+// halo means fewer launches but more redundant work per node: it pays for a
+// single run, whose launches are short; a batch of many runs, whose
+// launches fill the device many times over, takes the narrow one.
+constexpr int kJacH = 16;
+constexpr int kJacHNarrow = 8;
+constexpr int kJacNarrowMinBlocks = 1024;   // wide-halo workgroups from which to go narrow
+__host__ __device__ constexpr int jac_tx(int h) { return 64 - 2 * h; }
+__host__ __device__ constexpr int jac_ty(int h) { return kJacRows * kJacWaves - 2 * h; }
 
 enum Dir : int { kDirFar = 9, kDirProbe = 10, kDirTop = 11, kDirNone = 15 };
 

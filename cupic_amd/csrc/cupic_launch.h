@@ -30,7 +30,8 @@ enum Shape : int {
     kShapeInject,      // 2 * ceil(max_inject / 256)
     kShapeRows,        // 2 * ncy
     kShapeInit,        // 2 * ceil(n_init / 256)
-    kShapeJacBlk,      // ceil(ncx / kJacTX) * ceil((ncy - 1) / kJacTY)
+    kShapeJacBlk,      // ceil(ncx / jac_tx(kJacH)) * ceil((ncy - 1) / jac_ty(kJacH))
+    kShapeJacBlkN,     // the same for the narrow halo kJacHNarrow
     kShapeDftInit,     // ceil(ncx * 2K / 256) + ceil(K / 256)
     kShapeDftFwd,      // ceil((ncy + 1) / kGM) * ceil(2K / kGN)
     kShapeDftTri,      // ceil(2K / 256)
@@ -46,8 +47,8 @@ void launch_rho_convert(const Launch &L);
 void launch_solver_begin(const Launch &L, int use_args, int mode, double tol, int min_sweeps,
                          int max_sweeps, int check_every);
 void launch_jacobi_sweep(const Launch &L, int s);
-void launch_jacobi_block(const Launch &L, int s0, int nsw, int mode, int check_slot, int first,
-                         int par, int resid_slot);
+void launch_jacobi_block(const Launch &L, int halo, int s0, int nsw, int mode, int check_slot,
+                         int first, int par, int resid_slot);
 void launch_ir_update(const Launch &L, int par);
 void launch_residual(const Launch &L, int c, int mode, int par);
 void launch_solver_end(const Launch &L, int last_check, int par);

@@ -184,9 +184,9 @@ __device__ __forceinline__ float lane_next(float v)
                                                       true));
 }
 
-// nsw <= kJacH - 1 Jacobi sweeps s0 .. s0+nsw-1 in one launch (modes 1, 2).
-// A workgroup holds a 64 x (kJacWaves * kJacRows) node region -- a kJacTX x kJacTY
-// tile with a kJacH-node halo -- in registers: lane = region column, each
+// nsw <= H - 1 Jacobi sweeps s0 .. s0+nsw-1 in one launch (modes 1, 2).
+// A workgroup holds a 64 x (kJacWaves * kJacRows) node region -- a TX x TY
+// tile with an H-node halo -- in registers: lane = region column, each
 // wave kJacRows consecutive rows.  East/west neighbours come from the next
 // lanes, north/south from the thread's own rows; only the rows at wave
 // boundaries go through LDS.  The valid region shrinks by one node per
@@ -196,6 +196,7 @@ __device__ __forceinline__ float lane_next(float v)
 // d starting at 0 when first.  par selects the input buffer; the output goes
 // to the other one.  With resid_slot >= 0 (mode 1) the launch also reduces
 // the FP64 residual of its tile after the last sweep.
+template <int H>
 __global__ __launch_bounds__(kJacThreads) void k_jacobi_block(const RunDev *rds, const int *start,
                                                          int nruns, int s0, int nsw, int mode,
                                                          int check_slot, int first, int par,
@@ -203,6 +204,7 @@ __global__ __launch_bounds__(kJacThreads) void k_jacobi_block(const RunDev *rds,
 {
     RUN_PROLOGUE();
     constexpr int NR = kJacRows;
+    constexpr int kJacTX = jac_tx(H), kJacTY = jac_ty(H), kJacH = H;
     __shared__ float xch[2][kJacWaves][2][kWave];
     __shared__ double shr[kJacWaves];
     if (st->solve_done) return;
@@ -426,12 +428,18 @@ void launch_solver_begin(const Launch &L, int use_args, int mode, double tol, in
     LAUNCH_K(k_solver_begin, L, use_args, mode, tol, min_sweeps, max_sweeps, check_every);
 }
 void launch_jacobi_sweep(const Launch &L, int s) { LAUNCH_K(k_jacobi_sweep, L, s); }
-void launch_jacobi_block(const Launch &L, int s0, int nsw, int mode, int check_slot, int first,
-                         int par, int resid_slot)
+void launch_jacobi_block(const Launch &L, int halo, int s0, int nsw, int mode, int check_slot,
+                         int first, int par, int resid_slot)
 {
-    if (L.nblocks > 0)
-        hipLaunchKernelGGL(k_jacobi_block, dim3(L.nblocks), dim3(kJacThreads), 0, L.stream, L.rds,
-                           L.start, L.nruns, s0, nsw, mode, check_slot, first, par, resid_slot);
+    if (L.nblocks <= 0) return;
+    if (halo == kJacHNarrow)
+        hipLaunchKernelGGL(k_jacobi_block<kJacHNarrow>, dim3(L.nblocks), dim3(kJacThreads), 0,
+                           L.stream, L.rds, L.start, L.nruns, s0, nsw, mode, check_slot, first,
+                           par, resid_slot);
+    else
+        hipLaunchKernelGGL(k_jacobi_block<kJacH>, dim3(L.nblocks), dim3(kJacThreads), 0,
+                           L.stream, L.rds, L.start, L.nruns, s0, nsw, mode, check_slot, first,
+                           par, resid_slot);
 }
 void launch_ir_update(const Launch &L, int par) { LAUNCH_K(k_ir_update, L, par); }
 void launch_residual(const Launch &L, int c, int mode, int par)
