@@ -319,6 +319,8 @@ def main():
     ap.add_argument("--no-timing", action="store_true")
     ap.add_argument("--no-direct", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--tile", type=int, default=0,
+                    help="cells per wave strip in the push passes (1..16); skips the tuning")
     ap.add_argument("--batched-runs", type=int, default=256)
     ap.add_argument("--batched-steps", type=int, default=10)
     ap.add_argument("--serve-requests", type=int, default=12)
@@ -374,8 +376,10 @@ def main():
     for _ in range(a.warmup):
         run.step(t)
         t += p.dt
-    tile = 0
-    if not a.no_autotune:
+    tile = a.tile
+    if tile:
+        run.set_tile(tile)
+    elif not a.no_autotune:
         # in-situ: each candidate tile steps this run (untimed warm-up steps)
         t, tile, _, ttimes = run.autotune(t, steps=2)
         print(f"# autotuned tile {tile}: {ttimes}", file=sys.stderr, flush=True)
