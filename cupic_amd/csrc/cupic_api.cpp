@@ -103,7 +103,7 @@ int shape_blocks(const RunDev &h, Shape s)
     case kShapeCellBlk: return 2 * ((h.ncells + kScanCells - 1) / kScanCells);
     case kShapeRunSp: return 2;
     case kShapeRun: return 1;
-    case kShapeFar: return 2 * ((h.far_cap + kBlock - 1) / kBlock);
+    case kShapeFar: return 2 * far_blocks(h);
     case kShapeInject: return 2 * ((h.max_inject + kBlock - 1) / kBlock);
     case kShapeRows: return 2 * h.ncy;
     case kShapeInit: return 2 * ((h.n_init + kBlock - 1) / kBlock);

@@ -29,6 +29,7 @@ constexpr int kCmax = 16;              // max cells per wave task
 constexpr int kTileW = kCmax + 3;      // deposit tile width (node columns)
 constexpr int kScanCells = 1024;       // cells per scan block
 constexpr int kRedSlots = 16;         // atomic slots per reduced quantity
+constexpr int kFarBlocks = 16;         // far-list kernels: at most this many blocks per species
 constexpr int kGM = 64;        // direct solve: GEMM output tile rows (4 waves x 16)
 constexpr int kGN = 32;        // direct solve: GEMM output tile columns
 constexpr int kGK = 16;        // direct solve: GEMM k tile
@@ -145,6 +146,15 @@ struct RunDev {
     double *dft_z;                  // mode 3: [ncy+1][2K] transformed rows
     RunState *st;
 };
+
+// Blocks per species of the far-list kernels: grid-stride loops over the
+// entries, so a run with a large capacity and a short list launches few
+// idle blocks.
+__host__ __device__ inline int far_blocks(const RunDev &h)
+{
+    int b = (h.far_cap + kBlock - 1) / kBlock;
+    return b < kFarBlocks ? b : kFarBlocks;
+}
 
 // ---------------------------------------------------------------- helpers
 

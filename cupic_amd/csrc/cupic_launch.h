@@ -26,7 +26,7 @@ enum Shape : int {
     kShapeCellBlk,     // 2 * ceil(ncells / kScanCells)
     kShapeRunSp,       // 2: one block per species
     kShapeRun,         // 1
-    kShapeFar,         // 2 * ceil(far_cap / 256)
+    kShapeFar,         // 2 * far_blocks(): grid-stride over the far list
     kShapeInject,      // 2 * ceil(max_inject / 256)
     kShapeRows,        // 2 * ncy
     kShapeInit,        // 2 * ceil(n_init / 256)

@@ -269,17 +269,17 @@ __global__ __launch_bounds__(kBlock) void k_far_place(const RunDev *rds, const i
                                                       int nruns)
 {
     RUN_PROLOGUE();
-    int bps = (R.far_cap + kBlock - 1) / kBlock;
-    int s = local / bps;
+    const int bps = far_blocks(R);
+    const int s = local / bps;
     if (s > 1) return;
-    int e = (local - s * bps) * kBlock + threadIdx.x;
-    int F = min(st->far_n[s] + st->n_in[s], R.far_cap);
-    if (e >= F) return;
-    int key = R.far[s][e].key;
-    int slot = atomicAdd(&R.far_cur[s][key], 1);
-    int pos = R.base[s][9 * R.ncells + key] + slot;
-    if ((unsigned)pos < (unsigned)R.cap)
-        ((int *)R.part[s][st->cur ^ 1])[4 * pos] = e;  // first word of the slot
+    const int F = min(st->far_n[s] + st->n_in[s], R.far_cap);
+    for (int e = (local - s * bps) * kBlock + threadIdx.x; e < F; e += bps * kBlock) {
+        int key = R.far[s][e].key;
+        int slot = atomicAdd(&R.far_cur[s][key], 1);
+        int pos = R.base[s][9 * R.ncells + key] + slot;
+        if ((unsigned)pos < (unsigned)R.cap)
+            ((int *)R.part[s][st->cur ^ 1])[4 * pos] = e;  // first word of the slot
+    }
 }
 
 // Stable rank of a far entry within its cell: entries of the same cell with
@@ -288,23 +288,23 @@ __global__ __launch_bounds__(kBlock) void k_far_rank(const RunDev *rds, const in
                                                      int nruns)
 {
     RUN_PROLOGUE();
-    int bps = (R.far_cap + kBlock - 1) / kBlock;
-    int s = local / bps;
+    const int bps = far_blocks(R);
+    const int s = local / bps;
     if (s > 1) return;
-    int e = (local - s * bps) * kBlock + threadIdx.x;
-    int F = min(st->far_n[s] + st->n_in[s], R.far_cap);
-    if (e >= F) return;
-    const FarRec *far = R.far[s];
-    int key = far[e].key, idx = far[e].idx;
-    int b = R.base[s][9 * R.ncells + key];
-    int n = R.off[s][st->cur ^ 1][key + 1] - b;
-    const int *slots = (const int *)R.part[s][st->cur ^ 1];
-    int rank = 0;
-    for (int u = 0; u < n; u++) {
-        int q = slots[4 * (b + u)];
-        if ((unsigned)q < (unsigned)F) rank += far[q].idx < idx;
+    const int F = min(st->far_n[s] + st->n_in[s], R.far_cap);
+    for (int e = (local - s * bps) * kBlock + threadIdx.x; e < F; e += bps * kBlock) {
+        const FarRec *far = R.far[s];
+        int key = far[e].key, idx = far[e].idx;
+        int b = R.base[s][9 * R.ncells + key];
+        int n = R.off[s][st->cur ^ 1][key + 1] - b;
+        const int *slots = (const int *)R.part[s][st->cur ^ 1];
+        int rank = 0;
+        for (int u = 0; u < n; u++) {
+            int q = slots[4 * (b + u)];
+            if ((unsigned)q < (unsigned)F) rank += far[q].idx < idx;
+        }
+        R.far[s][e].rank = rank;
     }
-    R.far[s][e].rank = rank;
 }
 
 // Far-list entries to their sorted positions, with their charge (global
@@ -313,33 +313,33 @@ __global__ __launch_bounds__(kBlock) void k_far_scatter(const RunDev *rds, const
                                                         int nruns)
 {
     RUN_PROLOGUE();
-    int bps = (R.far_cap + kBlock - 1) / kBlock;
-    int s = local / bps;
+    const int bps = far_blocks(R);
+    const int s = local / bps;
     if (s > 1) return;
-    int e = (local - s * bps) * kBlock + threadIdx.x;
-    int F = min(st->far_n[s] + st->n_in[s], R.far_cap);
-    if (e >= F) return;
-    FarRec rec = R.far[s][e];
-    int o = st->cur ^ 1;
-    int dest = R.base[s][9 * R.ncells + rec.key] + rec.rank;
-    if ((unsigned)dest >= (unsigned)R.cap) {
-        atomicOr(&st->err, kErrSortInvariant);
-        return;
+    const int F = min(st->far_n[s] + st->n_in[s], R.far_cap);
+    for (int e = (local - s * bps) * kBlock + threadIdx.x; e < F; e += bps * kBlock) {
+        FarRec rec = R.far[s][e];
+        int o = st->cur ^ 1;
+        int dest = R.base[s][9 * R.ncells + rec.key] + rec.rank;
+        if ((unsigned)dest >= (unsigned)R.cap) {
+            atomicOr(&st->err, kErrSortInvariant);
+            continue;
+        }
+        st4(R.part[s][o], dest, v4f{rec.x, rec.y, rec.vx, rec.vy});
+        float fx, fy;
+        int ic, jc;
+        cell_of(R, rec.x, rec.y, fx, fy, ic, jc);
+        unsigned wx, wy;
+        fixed_weights(fx, fy, wx, wy);
+        long long sgn = s == 0 ? -1 : 1;
+        long long gx = 65536 - (long long)wx, gy = 65536 - (long long)wy;
+        int i1 = ic + 1 == R.ncx ? 0 : ic + 1;
+        atomicAdd(&R.acc[jc * R.ncx + ic], (unsigned long long)(sgn * gx * gy));
+        atomicAdd(&R.acc[jc * R.ncx + i1], (unsigned long long)(sgn * (long long)wx * gy));
+        atomicAdd(&R.acc[(jc + 1) * R.ncx + ic], (unsigned long long)(sgn * gx * (long long)wy));
+        atomicAdd(&R.acc[(jc + 1) * R.ncx + i1],
+                  (unsigned long long)(sgn * (long long)wx * (long long)wy));
     }
-    st4(R.part[s][o], dest, v4f{rec.x, rec.y, rec.vx, rec.vy});
-    float fx, fy;
-    int ic, jc;
-    cell_of(R, rec.x, rec.y, fx, fy, ic, jc);
-    unsigned wx, wy;
-    fixed_weights(fx, fy, wx, wy);
-    long long sgn = s == 0 ? -1 : 1;
-    long long gx = 65536 - (long long)wx, gy = 65536 - (long long)wy;
-    int i1 = ic + 1 == R.ncx ? 0 : ic + 1;
-    atomicAdd(&R.acc[jc * R.ncx + ic], (unsigned long long)(sgn * gx * gy));
-    atomicAdd(&R.acc[jc * R.ncx + i1], (unsigned long long)(sgn * (long long)wx * gy));
-    atomicAdd(&R.acc[(jc + 1) * R.ncx + ic], (unsigned long long)(sgn * gx * (long long)wy));
-    atomicAdd(&R.acc[(jc + 1) * R.ncx + i1],
-              (unsigned long long)(sgn * (long long)wx * (long long)wy));
 }
 
 // End of cc: flip buffers, commit counts and the probe tally (probe charge
