@@ -142,15 +142,16 @@ __global__ __launch_bounds__(kBlock) void k_inject(const RunDev *rds, const int 
 // Count of particles arriving in new cell (i, j) from neighbour slot sl
 // (source offset (sl%3-1, sl/3-1), periodic in x): the source cell's count
 // for the direction that leads here.  cnt is direction-major.
-__device__ __forceinline__ int src_count(const RunDev &R, const int *cnt, int i, int j, int sl)
+__device__ __forceinline__ int src_count(const CUPIC_GLOBAL int *cnt, int ncx, int ncy,
+                                         int ncells, int i, int j, int sl)
 {
     int dj = sl / 3 - 1, di = sl - 3 * (sl / 3) - 1;
     int jj = j + dj;
-    if (jj < 0 || jj >= R.ncy) return 0;
+    if (jj < 0 || jj >= ncy) return 0;
     int ii = i + di;
-    if (ii < 0) ii += R.ncx;
-    if (ii >= R.ncx) ii -= R.ncx;
-    return cnt[(8 - sl) * R.ncells + jj * R.ncx + ii];
+    if (ii < 0) ii += ncx;
+    if (ii >= ncx) ii -= ncx;
+    return cnt[(size_t)(8 - sl) * ncells + jj * ncx + ii];
 }
 
 // Exclusive scan of one value per thread over the block; returns the
@@ -178,29 +179,49 @@ __device__ __forceinline__ int block_excl_scan(int v, int *sh, int &total)
     return before + x - v;
 }
 
+// Block-wide sum of one value per thread (thread 0 gets it).
+__device__ __forceinline__ int block_sum(int v, int *sh)
+{
+    int lane = threadIdx.x & (kWave - 1), w = threadIdx.x >> 6;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, kWave);
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    int t = 0;
+#pragma unroll
+    for (int q = 0; q < kBlock / kWave; q++) t += sh[q];
+    return t;
+}
+
 // Per-cell totals of the new state (near arrivals plus the far list) and
-// per-block sums.  One thread per cell, 1024 cells per block.
+// per-block sums.  One thread per cell (kScanCells per block, interleaved by
+// kBlock so that every access is coalesced).
 __global__ __launch_bounds__(kBlock) void k_scan_a(const RunDev *rds, const int *start,
                                                    int nruns)
 {
     RUN_PROLOGUE();
     __shared__ int sh[kBlock / kWave];
-    int nblk = (R.ncells + kScanCells - 1) / kScanCells;
+    const int ncells = R.ncells, ncx = R.ncx, ncy = R.ncy;
+    int nblk = (ncells + kScanCells - 1) / kScanCells;
     int s = local / nblk;
     int blk = local - s * nblk;
+    const CUPIC_GLOBAL int *cnt = gp(R.cnt[s]);
+    const CUPIC_GLOBAL int *farc = gp(R.far_cnt[s]);
+    CUPIC_GLOBAL int *tot = gp(R.tot[s]);
     int sum = 0;
+#pragma unroll
     for (int u = 0; u < kScanCells / kBlock; u++) {
         int k = blk * kScanCells + u * kBlock + threadIdx.x;
-        if (k >= R.ncells) break;
-        int j = k / R.ncx, i = k - j * R.ncx;
-        int t = R.far_cnt[s][k];
+        if (k < ncells) {
+            int j = k / ncx, i = k - j * ncx;
+            int t = farc[k];
 #pragma unroll
-        for (int sl = 0; sl < 9; sl++) t += src_count(R, R.cnt[s], i, j, sl);
-        R.tot[s][k] = t;
-        sum += t;
+            for (int sl = 0; sl < 9; sl++) t += src_count(cnt, ncx, ncy, ncells, i, j, sl);
+            tot[k] = t;
+            sum += t;
+        }
     }
-    int total;
-    block_excl_scan(sum, sh, total);
+    int total = block_sum(sum, sh);
     if (threadIdx.x == 0) R.blk[s][blk] = total;
 }
 
@@ -232,34 +253,39 @@ __global__ __launch_bounds__(kBlock) void k_scan_c(const RunDev *rds, const int 
 {
     RUN_PROLOGUE();
     __shared__ int sh[kBlock / kWave];
-    int nblk = (R.ncells + kScanCells - 1) / kScanCells;
+    const int ncells = R.ncells, ncx = R.ncx, ncy = R.ncy;
+    int nblk = (ncells + kScanCells - 1) / kScanCells;
     int s = local / nblk;
     int blk = local - s * nblk;
-    int *noff = R.off[s][st->cur ^ 1];
-    int *base = R.base[s];
+    CUPIC_GLOBAL int *noff = gp(R.off[s][st->cur ^ 1]);
+    CUPIC_GLOBAL int *base = gp(R.base[s]);
+    const CUPIC_GLOBAL int *cnt = gp(R.cnt[s]);
+    const CUPIC_GLOBAL int *tot = gp(R.tot[s]);
+    CUPIC_GLOBAL int *farc = gp(R.far_cnt[s]);
+    CUPIC_GLOBAL int *farcur = gp(R.far_cur[s]);
     int carry = R.blk[s][blk];
     for (int u = 0; u < kScanCells / kBlock; u++) {
         int k = blk * kScanCells + u * kBlock + threadIdx.x;
-        bool ok = k < R.ncells;
-        int t = ok ? R.tot[s][k] : 0;
+        bool ok = k < ncells;
+        int t = ok ? tot[k] : 0;
         int total;
         int ex = block_excl_scan(t, sh, total);
         if (ok) {
-            int j = k / R.ncx, i = k - j * R.ncx;
+            int j = k / ncx, i = k - j * ncx;
             int b = carry + ex;
             noff[k] = b;
 #pragma unroll
             for (int sl = 0; sl < 9; sl++) {
-                base[sl * R.ncells + k] = b;
-                b += src_count(R, R.cnt[s], i, j, sl);
+                base[(size_t)sl * ncells + k] = b;
+                b += src_count(cnt, ncx, ncy, ncells, i, j, sl);
             }
-            base[9 * R.ncells + k] = b;
-            R.far_cnt[s][k] = 0;
-            R.far_cur[s][k] = 0;
+            base[(size_t)9 * ncells + k] = b;
+            farc[k] = 0;
+            farcur[k] = 0;
         }
         carry += total;
     }
-    if (blk == nblk - 1 && threadIdx.x == 0) noff[R.ncells] = st->N_new[s];
+    if (blk == nblk - 1 && threadIdx.x == 0) noff[ncells] = st->N_new[s];
 }
 
 // Far list, after the scan: each entry takes a place in its cell's far
