@@ -1234,6 +1234,18 @@ void batch_step_sequence(cupic_batch *b, const Target &T)
 
 extern "C" {
 
+// HIP failures while a batch is being built release what was built so far
+#define BATCH_TRY(expr)                                                                 \
+    do {                                                                                \
+        hipError_t e_ = (expr);                                                         \
+        if (e_ != hipSuccess) {                                                         \
+            if (b->stream) (void)hipStreamEndCapture(b->stream, &b->graph);             \
+            cupic_batch_destroy(b);                                                     \
+            return set_err(nullptr, CUPIC_ERR_HIP,                                      \
+                           std::string(#expr) + ": " + hipGetErrorString(e_));          \
+        }                                                                               \
+    } while (0)
+
 int cupic_batch_create(cupic_run *const *runs, int32_t nruns, int32_t use_graph,
                        cupic_batch **out)
 {
@@ -1248,12 +1260,12 @@ int cupic_batch_create(cupic_run *const *runs, int32_t nruns, int32_t use_graph,
     for (int k = 0; k < nruns; k++) {
         cupic_run *r = runs[k];
         if (!r || !r->initialised || r->moved) {
-            delete b;
+            cupic_batch_destroy(b);
             return set_err(nullptr, CUPIC_ERR_STATE, "batch runs must be initialised and idle");
         }
         if (r->p.jacobi_mode != b->mode || r->p.jacobi_check_every != b->check ||
             r->p.jacobi_max_sweeps != b->max_sweeps) {
-            delete b;
+            cupic_batch_destroy(b);
             return set_err(nullptr, CUPIC_ERR_ARG, "batch runs must share the Jacobi schedule");
         }
         b->runs.push_back(r);
@@ -1263,16 +1275,16 @@ int cupic_batch_create(cupic_run *const *runs, int32_t nruns, int32_t use_graph,
     // the coming step's t
     for (cupic_run *r : b->runs) {
         if (!r->acc_valid) {
-            HIPTRY(nullptr, hipMemsetAsync(r->h.acc, 0, (size_t)r->h.nnodes * 8, r->stream));
+            BATCH_TRY(hipMemsetAsync(r->h.acc, 0, (size_t)r->h.nnodes * 8, r->stream));
             launch_deposit(target_of(r).L(kShapeTasks));
             r->acc_valid = true;
         }
         RunState s;
         int st = read_state(r, &s);
-        if (st) { delete b; return st; }
+        if (st) { cupic_batch_destroy(b); return st; }
         s.t = r->t_next;
         st = write_state(r, s);
-        if (st) { delete b; return st; }
+        if (st) { cupic_batch_destroy(b); return st; }
     }
     std::vector<int> plan((size_t)kShapeCount * (nruns + 1));
     for (int s = 0; s < kShapeCount; s++) {
@@ -1289,19 +1301,19 @@ int cupic_batch_create(cupic_run *const *runs, int32_t nruns, int32_t use_graph,
         cupic_batch_destroy(b);
         return set_err(nullptr, CUPIC_ERR_ALLOC, "batch allocation failed");
     }
-    HIPTRY(nullptr, hipMemcpy(b->d_rds, hs.data(), hs.size() * sizeof(RunDev), hipMemcpyHostToDevice));
-    HIPTRY(nullptr, hipMemcpy(b->d_plan, plan.data(), plan.size() * sizeof(int), hipMemcpyHostToDevice));
-    HIPTRY(nullptr, hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
-    HIPTRY(nullptr, hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking));
-    HIPTRY(nullptr, hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming));
-    HIPTRY(nullptr, hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming));
-    HIPTRY(nullptr, hipDeviceSynchronize());
+    BATCH_TRY(hipMemcpy(b->d_rds, hs.data(), hs.size() * sizeof(RunDev), hipMemcpyHostToDevice));
+    BATCH_TRY(hipMemcpy(b->d_plan, plan.data(), plan.size() * sizeof(int), hipMemcpyHostToDevice));
+    BATCH_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    BATCH_TRY(hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking));
+    BATCH_TRY(hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming));
+    BATCH_TRY(hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming));
+    BATCH_TRY(hipDeviceSynchronize());
     if (b->use_graph) {
         Target T = batch_target(b);
-        HIPTRY(&b->err, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        BATCH_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
         batch_step_sequence(b, T);
-        HIPTRY(&b->err, hipStreamEndCapture(b->stream, &b->graph));
-        HIPTRY(&b->err, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+        BATCH_TRY(hipStreamEndCapture(b->stream, &b->graph));
+        BATCH_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
     }
     *out = b;
     return CUPIC_OK;
