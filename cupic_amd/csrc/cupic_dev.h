@@ -32,7 +32,7 @@ constexpr int kRedSlots = 16;         // atomic slots per reduced quantity
 constexpr int kFarBlocks = 16;         // far-list kernels: at most this many blocks per species
 constexpr int kGM = 64;        // direct solve: GEMM output tile rows (4 waves x 16)
 constexpr int kGN = 32;        // direct solve: GEMM output tile columns
-constexpr int kGK = 16;        // direct solve: GEMM k tile
+constexpr int kGK = 64;        // direct solve: GEMM k tile
 constexpr int kJacRows = 8;            // blocked Jacobi: rows per wave
 constexpr int kJacWaves = 8;           // blocked Jacobi: waves per workgroup
 constexpr int kJacThreads = kJacWaves * 64;
