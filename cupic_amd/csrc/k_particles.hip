@@ -596,32 +596,40 @@ __device__ __forceinline__ void push_task(const RunDev &R, RunState *st, const T
                 v2 = (unsigned long long)wx * (unsigned long long)wy + (1ull << 48);
             }
             // stayers: per-lane sums while the wave stays in one old cell,
-            // reduced once per cell (or every 60000 particles); chunks that
-            // span cells use one segmented sum per cell; movers add their own
-            const int nstay = __popcll(__ballot(stay));
-            if (one_cell && cl0 == acc_cl && acc_n + nstay < 60000) {
-                if (stay) { acc1 += v1; acc2 += v2; }
-                acc_n += nstay;
-            } else {
-                if (acc_cl >= 0) {
-                    flush_stayers(tile, TW, acc_cl, acc1, acc2, sgn, lane);
+            // reduced once per cell (or every 60000 particles); a chunk that
+            // spans cells feeds each cell segment in turn; movers add their own
+            const unsigned long long stay_m = __ballot(stay);
+            if (one_cell) {
+                const int ns = __popcll(stay_m);
+                if (cl0 != acc_cl || acc_n + ns >= 60000) {
+                    if (acc_cl >= 0) flush_stayers(tile, TW, acc_cl, acc1, acc2, sgn, lane);
                     acc1 = 0ull;
                     acc2 = 0ull;
-                    acc_cl = -1;
+                    acc_cl = cl0;
                     acc_n = 0;
                 }
-                if (one_cell) {
-                    acc_cl = cl0;
-                    acc_n = nstay;
-                    if (stay) { acc1 = v1; acc2 = v2; }
-                } else {
-                    unsigned long long s1 = seg_scan_u64(stay ? v1 : 0ull, lane, seg_start);
-                    unsigned long long s2 = seg_scan_u64(stay ? v2 : 0ull, lane, seg_start);
-                    unsigned long long nxt_head = heads & ~upto;
-                    bool seg_last = valid && (nxt_head ? (__ffsll((long long)nxt_head) - 1 == lane + 1)
-                                                       : (i + 1 >= p1 || lane == kWave - 1));
-                    if (seg_last && (s2 >> 48) != 0ull && in_strip)
-                        tile_add(tile, TW, 1 * TW + (cl + 1), s1, s2, sgn);
+                if (stay) { acc1 += v1; acc2 += v2; }
+                acc_n += ns;
+            } else {
+                unsigned long long rem = heads;
+                while (rem) {
+                    const int h = __ffsll((long long)rem) - 1;
+                    rem &= rem - 1ull;
+                    const unsigned long long end =
+                        rem ? ((1ull << (__ffsll((long long)rem) - 1)) - 1ull) : ~0ull;
+                    const unsigned long long M = end & ~((1ull << h) - 1ull);
+                    const int c = __builtin_amdgcn_readlane(cl, h);
+                    if (c < 0 || c >= nc) continue;  // outside the strip: no stayers
+                    const int ns = __popcll(stay_m & M);
+                    if (c != acc_cl || acc_n + ns >= 60000) {
+                        if (acc_cl >= 0) flush_stayers(tile, TW, acc_cl, acc1, acc2, sgn, lane);
+                        acc1 = 0ull;
+                        acc2 = 0ull;
+                        acc_cl = c;
+                        acc_n = 0;
+                    }
+                    if (stay && ((M >> lane) & 1ull)) { acc1 += v1; acc2 += v2; }
+                    acc_n += ns;
                 }
             }
             if (dir < 9 && !stay) {
