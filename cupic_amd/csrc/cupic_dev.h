@@ -7,7 +7,9 @@
 //   grids       periodic in x, [ncy+1][ncx]: node (i, j) at j*ncx + i.
 //               charge: int64 fixed point (2^-32 e); rhs: FP32 and FP64;
 //               phi: FP32 ping-pong (modes 0, 1) / FP64 master + FP32
-//               correction ping-pong (mode 2); E: float2 (Ex, Ey).
+//               correction ping-pong (mode 2) / FP64 (mode 3, direct);
+//               E: float2 (Ex, Ey).
+//   direct      mode 3 only: DFT bases, Thomas multipliers, transformed rows.
 //   sort meta   cnt[old cell][9 directions], base[new cell][10 slots],
 //               far list of particles that moved more than one cell or were
 //               injected.
@@ -25,16 +27,16 @@ namespace cupic {
 
 constexpr int kWave = 64;
 constexpr int kBlock = 256;
-constexpr int kCmax = 16;              // max cells per wave task
-constexpr int kTileW = kCmax + 3;      // deposit tile width (node columns)
-constexpr int kScanCells = 1024;       // cells per scan block
-constexpr int kRedSlots = 16;         // atomic slots per reduced quantity
-constexpr int kFarBlocks = 16;         // far-list kernels: at most this many blocks per species
-constexpr int kGM = 64;        // direct solve: GEMM output tile rows (4 waves x 16)
-constexpr int kGN = 32;        // direct solve: GEMM output tile columns
-constexpr int kGK = 64;        // direct solve: GEMM k tile
-constexpr int kJacRows = 8;            // blocked Jacobi: rows per wave
-constexpr int kJacWaves = 8;           // blocked Jacobi: waves per workgroup
+constexpr int kCmax = 16;           // max cells per wave task (tile_cells)
+constexpr int kTileW = kCmax + 3;   // deposit tile width (node columns)
+constexpr int kScanCells = 1024;    // cells per scan block
+constexpr int kRedSlots = 16;       // atomic slots per reduced quantity
+constexpr int kFarBlocks = 16;      // far-list kernels: at most this many blocks per species
+constexpr int kGM = 64;             // direct solve: GEMM output tile rows (4 waves x 16)
+constexpr int kGN = 32;             // direct solve: GEMM output tile columns
+constexpr int kGK = 64;             // direct solve: GEMM k tile
+constexpr int kJacRows = 8;         // blocked Jacobi: rows per wave
+constexpr int kJacWaves = 8;        // blocked Jacobi: waves per workgroup
 constexpr int kJacThreads = kJacWaves * 64;
 // Halo of the blocked Jacobi, at most halo - 1 sweeps per launch.  A wide
 // halo means fewer launches but more redundant work per node: it pays for a
