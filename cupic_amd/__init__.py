@@ -23,6 +23,7 @@ ERRORS = {-1: "ARG", -2: "HIP", -3: "ALLOC", -4: "CAPACITY", -5: "STATE", -6: "N
           -7: "IO"}
 
 JACOBI_REFERENCE, JACOBI_RESIDUAL, JACOBI_REFINE, POISSON_DIRECT = 0, 1, 2, 3
+HINTS_OFF, HINTS_ON, HINTS_CHECKED = 0, 1, 2
 QUIRK_Q7_REFERENCE_KICK = 1
 MESH_RHO, MESH_PHI, MESH_EX, MESH_EY, MESH_CHARGE = 0, 1, 2, 3, 4
 ELECTRONS, IONS = 0, 1
@@ -136,6 +137,7 @@ def _load():
         "cupic_autotune_tile": [P, C.c_int32, i32p, dp, dp],
         "cupic_autotune_run": [vp, dp, C.c_int32, i32p, dp, dp],
         "cupic_run_set_tile": [vp, C.c_int32],
+        "cupic_run_set_hint_mode": [vp, C.c_int32],
         "cupic_run_params": [vp, P],
         "cupic_batch_launch": [vp, C.c_int32],
         "cupic_batch_wait": [vp],
@@ -369,6 +371,11 @@ class Run:
         """Cells per wave strip of the push passes (0: from ppc); speed only."""
         self._c(lib.cupic_run_set_tile(self.h, tile))
         self.p.tile_cells = tile
+
+    def set_hint_mode(self, mode):
+        """Push hints of the counting pass: HINTS_OFF, HINTS_ON (default) or
+        HINTS_CHECKED (every hint verified against the push); speed only."""
+        self._c(lib.cupic_run_set_hint_mode(self.h, mode))
 
     def autotune(self, t, steps=3):
         """Tunes the tile on this run by stepping it from time t; returns

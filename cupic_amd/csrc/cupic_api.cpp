@@ -344,6 +344,7 @@ int read_state(cupic_run *r, RunState *out)
         if (out->err & kErrCapacity) m += " particle capacity exceeded;";
         if (out->err & kErrInjectCap) m += " injection count above max_inject;";
         if (out->err & kErrSortInvariant) m += " sort invariant broken;";
+        if (out->err & kErrHint) m += " push hint contradicted by the push;";
         return set_err(&r->err, CUPIC_ERR_CAPACITY, m);
     }
     return CUPIC_OK;
@@ -570,6 +571,18 @@ int cupic_run_set_tile(cupic_run *r, int32_t tile)
     return CUPIC_OK;
 }
 
+int cupic_run_set_hint_mode(cupic_run *r, int32_t mode)
+{
+    if (!r) return set_err(nullptr, CUPIC_ERR_ARG, "null run");
+    if (mode < kHintOff || mode > kHintChecked)
+        return set_err(&r->err, CUPIC_ERR_ARG, "hint mode must be 0, 1 or 2");
+    if (r->moved) return set_err(&r->err, CUPIC_ERR_STATE, "set_hint_mode between mover and cc");
+    HIPTRY(&r->err, hipStreamSynchronize(r->stream));
+    r->h.hint_mode = mode;
+    HIPTRY(&r->err, hipMemcpy(r->d_rd, &r->h, sizeof(RunDev), hipMemcpyHostToDevice));
+    return CUPIC_OK;
+}
+
 int cupic_run_footprint(const cupic_params *pin, int64_t capacity_hint, int64_t *bytes)
 {
     if (!pin || !bytes) return set_err(nullptr, CUPIC_ERR_ARG, "null argument");
@@ -579,6 +592,7 @@ int cupic_run_footprint(const cupic_params *pin, int64_t capacity_hint, int64_t 
     Sizes z = sizes_of(&p, capacity_hint);
     int64_t b = 0;
     b += 2 * 2 * 4 * z.cap * 4;                  // particles
+    b += 2 * 2 * ((z.cap + 255) & ~(int64_t)255);  // push hints
     b += 2 * 2 * (z.ncells + 1) * 4;             // offsets
     b += 2 * z.ncells * (9 + 10 + 1 + 1 + 1) * 4;  // cnt, base, far_cnt, far_cur, tot
     b += 2 * z.far_cap * (int64_t)sizeof(FarRec) + 2 * z.max_inject * 8;
@@ -650,10 +664,13 @@ int cupic_run_create(const cupic_params *pin, int64_t capacity_hint, cupic_run *
     h.seed = p.seed;
     h.run_id = p.run_id;
     h.quirks = p.quirks;
+    h.hint_mode = kHintOn;
     int st = CUPIC_OK;
     for (int s = 0; s < 2 && !st; s++) {
         for (int b = 0; b < 2; b++) {
             h.part[s][b] = dev_alloc<float4>(r, z.cap, &st);
+            // padded to whole 256-byte lines: the counting pass reads 4-byte words
+            h.hint[s][b] = dev_alloc<unsigned char>(r, ((size_t)z.cap + 255) & ~(size_t)255, &st);
             h.off[s][b] = dev_alloc<int>(r, z.ncells + 1, &st);
         }
         h.cnt[s] = dev_alloc<int>(r, z.ncells * 9, &st);
@@ -986,6 +1003,7 @@ int cupic_set_particles(cupic_run *r, int species, const float *x, const float *
                               hipMemcpyHostToDevice));
     HIPTRY(&r->err, hipMemcpy(h.off[species][s.cur], off.data(), off.size() * sizeof(int),
                               hipMemcpyHostToDevice));
+    HIPTRY(&r->err, hipMemset(h.hint[species][s.cur], 0, (size_t)std::max<int64_t>(n, 1)));
     s.N[species] = (int)n;
     s.N_new[species] = (int)n;
     r->acc_valid = false;

@@ -3,7 +3,8 @@
 // Layout in HBM (per run):
 //   particles   FP32 (x, y, vx, vy) as one 16-byte record per particle (AoS),
 //               per species, two buffers (ping-pong), sorted by cell key
-//               jc*ncx + ic with per-cell offsets.
+//               jc*ncx + ic with per-cell offsets; beside each buffer one
+//               push-hint byte per particle slot.
 //   grids       periodic in x, [ncy+1][ncx]: node (i, j) at j*ncx + i.
 //               charge: int64 fixed point (2^-32 e); rhs: FP32 and FP64;
 //               phi: FP32 ping-pong (modes 0, 1) / FP64 master + FP32
@@ -100,7 +101,17 @@ enum : unsigned int {
     kErrCapacity = 2u,
     kErrInjectCap = 4u,
     kErrSortInvariant = 8u,
+    kErrHint = 16u,                 // checked hint mode: a hint disagreed with the push
 };
+
+// Push hints (one byte per particle slot, written with the particle by the
+// scatter): low 4 bits the direction 0..8 the particle will move if its
+// velocity does not change, high 4 bits a code c >= 1 stating that the
+// position it would reach lies at least 2^(c-16) cells from every wall of
+// that cell; 0 means no hint.  The counting pass takes the direction from
+// the hint whenever the strip's field cannot move the particle by that
+// margin.
+enum : int { kHintOff = 0, kHintOn = 1, kHintChecked = 2 };
 
 // Immutable per-run description (device memory).
 struct RunDev {
@@ -120,7 +131,9 @@ struct RunDev {
     SpecConst sp[2];
     unsigned long long seed, run_id;
     unsigned int quirks;
+    int hint_mode;                  // kHintOff / kHintOn / kHintChecked
     float4 *part[2][2];             // [species][buffer], (x, y, vx, vy) per particle
+    unsigned char *hint[2][2];      // [species][buffer], push hint per particle slot
     int *off[2][2];                 // [species][buffer], ncells + 1
     int *cnt[2];                    // [9][ncells] arrivals per (direction, old cell)
     int *base[2];                   // [10][ncells] slot bases per new cell
@@ -280,6 +293,7 @@ __device__ __forceinline__ double u01_open(unsigned long long r)
 struct Geo {
     int ncx, ncy;
     float inv_ds, Lx_f, Ly_f, dt_f;
+    float dtinv;                    // dt / ds in FP32 (push hints)
 };
 
 __device__ __forceinline__ Geo geo_of(const RunDev &R)
@@ -291,6 +305,7 @@ __device__ __forceinline__ Geo geo_of(const RunDev &R)
     g.Lx_f = R.Lx_f;
     g.Ly_f = R.Ly_f;
     g.dt_f = R.dt_f;
+    g.dtinv = R.dt_f * R.inv_ds;
     return g;
 }
 
@@ -403,6 +418,58 @@ __device__ __forceinline__ void fixed_weights(float fx, float fy, unsigned &wx, 
 {
     wx = (unsigned)rintf(fx * 65536.0f);
     wy = (unsigned)rintf(fy * 65536.0f);
+}
+
+// ---------------------------------------------------------------- push hints
+//
+// Bound behind the hints.  A particle stored in cell (ic, jc) with in-cell
+// fraction fx = tx - ic (exact: tx = fl(x inv_ds), Sterbenz) and velocity vx
+// gets the predicted fraction F = fl(fma(vx, fl(dt inv_ds), fx)).  Its next
+// push computes vx' = fl(fma(qdtm, ex, vx)), x' = fl(fma(dt, vx', x)),
+// tx' = fl(x' inv_ds) and the cell floor(tx').  With u = 2^-24, positions
+// inside a grid of at most M = max(ncx, ncy) cells and |F| < 2,
+//   |(tx' - ic) - F| <= |dt qdtm inv_ds| |ex| + (3.2 M + 7) u,
+// and |ex| <= (1 + 7u) max |E| over the four gather nodes.  The hint's
+// distance m of F to the walls of its cell floor(F) is exact up to u
+// (F - floor(F) is exact but for -1/2 < F < 0, 1 - that within u/2), so
+// m >= 2^(c-16) - u.  Hence, with
+//   thr >= |dt qdtm inv_ds| Emax (1 + 2^-14) + 16 (M + 1) u,
+// a code with 2^(c-16) > thr puts tx' strictly inside the hinted cell: the
+// push lands there, with no wrap in x (hinted columns 1 .. ncx-2) and no
+// absorption in y (rows 0 .. ncy-1).  A non-finite F gives no hint; a
+// non-finite field gives no threshold.
+
+// Hint of a particle stored in cell (ic, jc) with fractions (fx, fy) and
+// velocity (vx, vy).
+template <class G>
+__device__ __forceinline__ unsigned push_hint(const G &R, float fx, float fy, float vx, float vy,
+                                              int ic, int jc)
+{
+    const float fxp = fmaf(vx, R.dtinv, fx), fyp = fmaf(vy, R.dtinv, fy);
+    if (!(fxp >= -1.0f && fxp < 2.0f && fyp >= -1.0f && fyp < 2.0f)) return 0u;  // NaN: no hint
+    const float kx = floorf(fxp), ky = floorf(fyp);
+    const int di = (int)kx, dj = (int)ky;
+    const int gi = ic + di, gj = jc + dj;
+    if (gi < 1 || gi > R.ncx - 2 || gj < 0 || gj > R.ncy - 1) return 0u;
+    const float gx = fxp - kx, gy = fyp - ky;
+    const float m = fminf(fminf(gx, 1.0f - gx), fminf(gy, 1.0f - gy));
+    // floor(log2 m) + 16 from the exponent field (0 <= m <= 1/2; zero and
+    // subnormal m give a negative code)
+    const int c = (int)((__float_as_uint(m) >> 23) & 0xFFu) - 127 + 16;
+    if (c < 1) return 0u;
+    return ((unsigned)min(c, 15) << 4) | (unsigned)((di + 1) + 3 * (dj + 1));
+}
+
+// Smallest hint code the counting pass may trust for a strip whose field
+// nodes have max(|Ex|, |Ey|) = emax (16: none).
+__device__ __forceinline__ int hint_code_needed(float qdtm, float dt, float inv_ds, int ncx,
+                                                int ncy, float emax, bool finite)
+{
+    const float a = fabsf(qdtm) * dt * inv_ds * emax * (1.0f + 0x1p-14f);
+    const float thr = a + 16.0f * (float)(max(ncx, ncy) + 1) * 0x1p-24f;
+    if (!finite || !(thr < 0.25f)) return 16;
+    const int e = (int)((__float_as_uint(thr) >> 23) & 0xFFu) - 127;  // thr is normal here
+    return max(1, e + 17);
 }
 
 }  // namespace cupic

@@ -352,6 +352,7 @@ __global__ __launch_bounds__(kBlock) void k_far_scatter(const RunDev *rds, const
             continue;
         }
         st4(R.part[s][o], dest, v4f{rec.x, rec.y, rec.vx, rec.vy});
+        R.hint[s][o][dest] = 0;  // placed without a hint: pushed in full next step
         float fx, fy;
         int ic, jc;
         cell_of(R, rec.x, rec.y, fx, fy, ic, jc);

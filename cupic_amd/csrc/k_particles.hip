@@ -1,19 +1,21 @@
 // k_particles.hip -- particle kernels.
 //
-// The push is evaluated twice per step from the same FP32 inputs, once to
-// count and once to scatter, which makes the stable cell-key counting sort
-// cost one extra read of the particle state instead of a key/permutation
-// round trip:
+// The stable cell-key counting sort of each step needs the number of
+// particles per (old cell, direction) before any particle can be stored:
 //
-//   pass 1 (k_push_rank): push, wrap, absorb, classify each particle by the
-//     direction it moved (9 neighbour cells, far, probe, top) and rank it
-//     within (old cell, direction); write per-cell direction counts.  Each
-//     wave owns a strip of whole old cells, so ranks are deterministic.
+//   pass 1 (k_push_rank): classify each particle by the direction it moves
+//     (9 neighbour cells, far, probe, top) and count per (old cell,
+//     direction); write the counts, the far list and the absorption
+//     tallies.  The direction comes from the particle's one-byte push hint
+//     whenever the strip's field is too weak to overturn it, so most
+//     particles cost one byte here; the rest are loaded and pushed.
 //   scan (k_cc.hip): new-cell offsets and per-slot bases.
-//   pass 2 (k_scatter): the identical push and ranks, then a store of the
-//     new state at its sorted position and the int64 fixed-point CIC charge
-//     of the new positions (the next step's deposition), reduced per wave
-//     by cell in registers and per wave-strip in LDS before HBM atomics.
+//   pass 2 (k_scatter): the push of every particle and its rank within
+//     (old cell, direction), then a store of the new state and its hint at
+//     the sorted position and the int64 fixed-point CIC charge of the new
+//     positions (the next step's deposition), reduced per wave by cell in
+//     registers and per wave-strip in LDS before HBM atomics.  Each wave
+//     owns a strip of whole old cells, so ranks are deterministic.
 //
 // Order within a new cell: by source slot (the 3x3 neighbourhood of old
 // cells in row-major order, then far movers and injected particles), and by
@@ -134,6 +136,7 @@ __global__ __launch_bounds__(kBlock) void k_init_sort(const RunDev *rds, const i
             for (int t = threadIdx.x + 1; t < kBlock; t++) after += (chunk[t] == ic);
             int dest = first + hist[ic] + runc[ic] + rank;
             st4(OUT, dest, p);
+            R.hint[s][0][dest] = 0;
         }
         __syncthreads();
         if (valid && after == 0) runc[ic] += rank + 1;
@@ -207,7 +210,9 @@ struct TaskInfo {
 __device__ __forceinline__ TaskInfo task_of(const RunDev &R, int local)
 {
     TaskInfo t;
-    int task = local * (kBlock / kWave) + (threadIdx.x >> 6);
+    // the wave index is uniform: keep the task and everything derived from
+    // it in scalar registers
+    int task = local * (kBlock / kWave) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     int per_species = R.ncy * R.tpr;
     t.ok = task < 2 * per_species;
     t.s = task / per_species;
@@ -288,6 +293,7 @@ __global__ __launch_bounds__(kBlock) void k_half_kick(const RunDev *rds, const i
         p.z = fmaf(hk, ex, p.z);
         p.w = fmaf(hk, ey, p.w);
         st4(P, i, p);
+        R.hint[s][cur][i] = 0;
     }
 }
 
@@ -425,19 +431,23 @@ __device__ __forceinline__ Pushed push_tile(const Geo &R, const float2 *Et, int 
     return o;
 }
 
-// One wave processes the particles of a strip of whole old cells.  Pass 1
-// (kScatter false) counts arrivals per (old cell, direction) and appends
-// far movers; pass 2 replays the identical push and ranks with each counter
-// started at its slot base, stores every near particle at its sorted
-// position and deposits its charge.
-template <bool kScatter>
-__device__ __forceinline__ void push_task(const RunDev &R, RunState *st, const TaskInfo &T,
-                                          int *cntw, int *gonew, float2 *Et,
-                                          unsigned long long *tile, int lane)
+constexpr int kQueue = 512;  // pass 1: ring of slots awaiting a full push, per wave
+
+// Pass 1: one wave per strip of whole old cells counts arrivals per (old
+// cell, direction), appends far movers to the far list and tallies
+// absorptions.  A particle whose hint the strip's field cannot overturn is
+// counted from its hint byte alone (movers need their old cell, found from
+// the strip's cell offsets).  The slots of the others are queued in LDS and
+// pushed 64 at a time, so the full push runs on full waves.  With
+// kHintChecked every particle is pushed and each trusted hint is compared
+// with its push.
+__device__ __forceinline__ void count_task(const RunDev &R, RunState *st, const TaskInfo &T,
+                                           int *cntw, int *gonew, int *bnd, int *queue,
+                                           float2 *Et, int lane)
 {
     const int s = T.s, j0 = T.j0, col0 = T.col0, nc = T.nc;
     const Geo G = geo_of(R);
-    const int cap = R.cap, far_cap = R.far_cap;
+    const int far_cap = R.far_cap;
     int *const far_cnt = R.far_cnt[s];
     FarRec *const farl = R.far[s];
     const int cur = st->cur;
@@ -445,14 +455,197 @@ __device__ __forceinline__ void push_task(const RunDev &R, RunState *st, const T
     const int c0 = j0 * R.ncx + col0;
     const int p0 = off[c0], p1 = off[c0 + nc];
     const float4 *PIN = R.part[s][cur];
+    const CUPIC_GLOBAL unsigned char *HIN = gp(R.hint[s][cur]);
+    const float qdtm = R.sp[s].qdtm;
+    const int mode = R.hint_mode;
+    // E tile of the strip and its largest component
+    float em = 0.0f;
+    bool fin = true;
+    {
+        const int w = nc + 1;
+        for (int q = lane; q < 2 * w; q += kWave) {
+            int r = q / w, c = col0 + (q - r * w);
+            if (c >= G.ncx) c -= G.ncx;
+            float2 e = ld_f2(R.E, (j0 + r) * G.ncx + c);
+            Et[q] = e;
+            float ax = fabsf(e.x), ay = fabsf(e.y);
+            fin = fin && ax < INFINITY && ay < INFINITY;  // false for NaN
+            em = fmaxf(em, fmaxf(ax, ay));
+        }
+    }
+    em = wave_max_f(em);
+    const int c_need =
+        hint_code_needed(qdtm, G.dt_f, G.inv_ds, G.ncx, G.ncy, em, __ballot(!fin) == 0ull);
+    // a hint byte at or above this value is trusted (code in the high nibble)
+    const unsigned h_min = mode == kHintOn ? (unsigned)c_need << 4 : 0x100u;
+    for (int q = lane; q < nc * 9; q += kWave) cntw[q] = 0;
+    for (int q = lane; q < nc; q += kWave) gonew[q] = 0;
+    for (int q = lane; q <= nc; q += kWave) bnd[q] = off[c0 + q];
+    wave_lds_sync();
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    unsigned long long probe_n = 0, top_n = 0;
+    bool bad = false, wrong = false;
+
+    // full push of queued slot i (lanes with ok false take no part)
+    auto push_full = [&](int i, bool ok) {
+        int dir = kDirNone;
+        Pushed o = {};
+        if (ok) {
+            const v4f p = ld4(PIN, i);
+            bool in_strip;
+            o = push_tile(G, Et, nc, col0, j0, R.E, qdtm, p.x, p.y, p.z, p.w, in_strip);
+            dir = o.dir;
+            if (dir < 9 && !in_strip) {  // not in this strip: the sort invariant is broken
+                dir = kDirFar;
+                bad = true;
+            }
+            const int cl = o.oic - col0;
+            if (dir != 4 && cl >= 0 && cl < nc) {
+                if (dir < 9) atomicAdd(&cntw[cl * 9 + dir], 1);
+                else atomicAdd(&gonew[cl], 1);
+            }
+            if (mode == kHintChecked) {
+                const unsigned h = HIN[i];
+                if ((int)(h >> 4) >= c_need && dir != (int)(h & 15u)) wrong = true;
+            }
+        }
+        const unsigned long long fm = __ballot(dir == kDirFar);
+        if (fm) {
+            int leader = __ffsll((long long)fm) - 1;
+            int fb = 0;
+            if (lane == leader) fb = atomicAdd(&st->far_n[s], __popcll(fm));
+            fb = __shfl(fb, leader, kWave);
+            if (dir == kDirFar) {
+                int slot = fb + __popcll(fm & lt);
+                if (slot < far_cap) {
+                    CUPIC_GLOBAL FarRec *f = gp(farl) + slot;
+                    int key = o.njc * G.ncx + o.nic;
+                    f->key = key;
+                    f->idx = i;
+                    f->rank = 0;
+                    f->pad = 0;
+                    f->x = o.x; f->y = o.y; f->vx = o.vx; f->vy = o.vy;
+                    atomicAdd(&far_cnt[key], 1);
+                } else {
+                    atomicOr(&st->err, kErrFarOverflow);
+                }
+            }
+        }
+        probe_n += __popcll(__ballot(dir == kDirProbe));
+        top_n += __popcll(__ballot(dir == kDirTop));
+    };
+
+    // Hints are read as words of four slots, lane l covering slots
+    // w + 4l .. w + 4l + 3 of the 256-slot window starting at w; the words of
+    // the next window are in flight while one is processed.
+    const int p0u = __builtin_amdgcn_readfirstlane(p0), p1u = __builtin_amdgcn_readfirstlane(p1);
+    // lane k: first slot of strip cell k (from LDS, so that no wait for HBM
+    // loads in the loop waits on the hint prefetch too)
+    const int bv = lane <= nc ? bnd[lane] : 0x7FFFFFFF;
+    const CUPIC_GLOBAL unsigned *HW = (const CUPIC_GLOBAL unsigned *)HIN;
+    int cb = 0;  // strip cell of the window's first slot
+    int qh = 0;  // queue: ring of kQueue slots, head qh, qn entries
+    int qn = 0;
+    auto window = [&](int w0, unsigned word) {
+        while (cb + 1 < nc && __builtin_amdgcn_readlane(bv, cb + 1) <= w0) cb++;
+        int cl[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) cl[q] = cb;
+        for (int k = cb + 1; k < nc; k++) {
+            const int bk = __builtin_amdgcn_readlane(bv, k);
+            if (bk >= w0 + 4 * kWave) break;
+#pragma unroll
+            for (int q = 0; q < 4; q++) cl[q] += (w0 + 4 * lane + q >= bk);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int i = w0 + 4 * lane + q;
+            const bool valid = i >= p0u && i < p1u;
+            const unsigned h = (word >> (8 * q)) & 0xFFu;
+            const bool use = valid && h >= h_min;
+            const int hdir = (int)(h & 15u);
+            if (use && hdir != 4) atomicAdd(&cntw[cl[q] * 9 + hdir], 1);
+            const bool full = valid && !use;
+            const unsigned long long fq = __ballot(full);
+            if (full) queue[(qh + qn + mbcnt64(fq)) & (kQueue - 1)] = i;
+            qn += __popcll(fq);
+        }
+        // at most kWave - 1 + 4 kWave entries: the ring never overfills
+        while (qn >= kWave) {
+            wave_lds_sync();
+            push_full(queue[(qh + lane) & (kQueue - 1)], true);
+            qh = (qh + kWave) & (kQueue - 1);
+            qn -= kWave;
+        }
+    };
+    // every lane loads a clamped, in-bounds word whatever the mode, so that
+    // the count of loads in flight is the same on every path and the wait
+    // for one window does not wait for the next; bytes of slots outside the
+    // strip, and all bytes unless the mode trusts hints, are ignored
+    auto load_word = [&](int w0) -> unsigned {
+        return HW[max(0, min(w0 + 4 * lane, p1u - 1)) >> 2];
+    };
+    const int a0 = p0u & ~3;
+    unsigned wa = load_word(a0);
+    for (int w0 = a0; w0 < p1u; w0 += 8 * kWave) {
+        const unsigned wb = load_word(w0 + 4 * kWave);
+        window(w0, wa);
+        wa = load_word(w0 + 8 * kWave);
+        if (w0 + 4 * kWave < p1u) window(w0 + 4 * kWave, wb);
+    }
+    if (qn > 0) {
+        wave_lds_sync();
+        push_full(lane < qn ? queue[(qh + lane) & (kQueue - 1)] : 0, lane < qn);
+    }
+    wave_lds_sync();
+    // stayers = population - movers - departures (probe, top, far)
+    for (int cl = lane; cl < nc; cl += kWave) {
+        int moved = gonew[cl];
+#pragma unroll
+        for (int d = 0; d < 9; d++)
+            if (d != 4) moved += cntw[cl * 9 + d];
+        cntw[cl * 9 + 4] = bnd[cl + 1] - bnd[cl] - moved;
+    }
+    wave_lds_sync();
+    CUPIC_GLOBAL int *cnt = gp(R.cnt[s]);
+    const int ncells = R.ncells;
+    for (int q = lane; q < nc * 9; q += kWave) {
+        int cl = q / 9, d = q - 9 * cl;
+        cnt[d * ncells + c0 + cl] = cntw[q];
+    }
+    if (lane == 0) {
+        if (probe_n) atomicAdd(&st->probe[s], probe_n);
+        if (top_n) atomicAdd(&st->top[s], top_n);
+    }
+    if (__ballot(bad) && lane == 0) atomicOr(&st->err, kErrSortInvariant);
+    if (__ballot(wrong) && lane == 0) atomicOr(&st->err, kErrHint);
+}
+
+// Pass 2: one wave per strip replays the identical push, ranks every near
+// particle within (old cell, direction) with each counter started at its
+// slot base, stores the particle and its hint at the sorted position and
+// deposits its charge.
+__device__ __forceinline__ void scatter_task(const RunDev &R, RunState *st, const TaskInfo &T,
+                                             int *cntw, float2 *Et, unsigned long long *tile,
+                                             int lane)
+{
+    const int s = T.s, j0 = T.j0, col0 = T.col0, nc = T.nc;
+    const Geo G = geo_of(R);
+    const int cap = R.cap;
+    const int cur = st->cur;
+    const CUPIC_GLOBAL int *off = gp(R.off[s][cur]);
+    const int c0 = j0 * R.ncx + col0;
+    const int p0 = off[c0], p1 = off[c0 + nc];
+    const float4 *PIN = R.part[s][cur];
     float4 *POUT = R.part[s][cur ^ 1];
+    CUPIC_GLOBAL unsigned char *HOUT = gp(R.hint[s][cur ^ 1]);
     const int ncells = R.ncells;
     const float2 *E = R.E;
     const float qdtm = R.sp[s].qdtm;
     const long long sgn = s == 0 ? -1 : 1;
     const int TW = nc + 3;
     load_E_tile(R, Et, j0, col0, nc, lane);
-    if (kScatter) {
+    {
         // counters start at the base of the slot each (cell, direction) fills
         const CUPIC_GLOBAL int *base = gp(R.base[s]);
         for (int q = lane; q < nc * 9; q += kWave) {
@@ -469,14 +662,10 @@ __device__ __forceinline__ void push_task(const RunDev &R, RunState *st, const T
             cntw[q] = v;
         }
         for (int q = lane; q < 4 * TW; q += kWave) tile[q] = 0ull;
-    } else {
-        for (int q = lane; q < nc * 9; q += kWave) cntw[q] = 0;
-        for (int q = lane; q < nc; q += kWave) gonew[q] = 0;
     }
     wave_lds_sync();
     const unsigned long long lt = (1ull << lane) - 1ull;
     const unsigned long long upto = lt | (1ull << lane);
-    unsigned long long probe_n = 0, top_n = 0;
     bool bad = false;
     unsigned long long acc1 = 0ull, acc2 = 0ull;  // stayer sums of old cell acc_cl
     int acc_cl = -1, acc_n = 0;
@@ -497,30 +686,17 @@ __device__ __forceinline__ void push_task(const RunDev &R, RunState *st, const T
             dir = kDirFar;
             bad = true;
         }
-        int rank = 0;
-        int seg_start = lane;
-        bool one_cell = false;
-        int cl0 = 0;
-        unsigned long long heads = 0ull;
-        if (!kScatter) {
-            // pass 1 needs counts only: movers and departures are counted per
-            // (old cell, direction) with LDS atomics (few lanes); stayers are
-            // the cell's population minus those, once per cell after the loop
-            if (valid && dir != 4 && cl >= 0 && cl < nc) {
-                if (dir < 9) atomicAdd(&cntw[cl * 9 + dir], 1);
-                else atomicAdd(&gonew[cl], 1);
-            }
-        } else {
         // rank within (old cell, direction); lanes of one old cell are contiguous
-        int clp = dpp_prev_int(cl);
-        bool head = valid && (lane == 0 || cl != clp);
-        heads = __ballot(head);
-        unsigned long long near_m = __ballot(dir < 9);
-        one_cell = (heads == 1ull);
+        int rank = 0;
+        int cl0 = 0;
+        const int clp = dpp_prev_int(cl);
+        const bool head = valid && (lane == 0 || cl != clp);
+        const unsigned long long heads = __ballot(head);
+        const unsigned long long near_m = __ballot(dir < 9);
+        const bool one_cell = (heads == 1ull);
         if (one_cell) {
             // the whole chunk lies in one old cell: per direction present,
             // rank = counter + earlier lanes with that direction
-            seg_start = 0;
             cl0 = __builtin_amdgcn_readfirstlane(cl);
             unsigned long long pend = near_m;
             while (pend) {
@@ -538,7 +714,7 @@ __device__ __forceinline__ void push_task(const RunDev &R, RunState *st, const T
                 unsigned long long m = __ballot(dir == dd);
                 if (dir == dd) mymask = m;
             }
-            seg_start = valid ? 63 - __clzll((long long)(heads & upto)) : lane;
+            const int seg_start = valid ? 63 - __clzll((long long)(heads & upto)) : lane;
             bool last = false;
             if (dir < 9) {
                 unsigned long long after = heads & ~upto;
@@ -551,119 +727,70 @@ __device__ __forceinline__ void push_task(const RunDev &R, RunState *st, const T
             wave_lds_sync();
             if (dir < 9 && last) cntw[cl * 9 + dir] = rank + 1;
         }
-        }
         wave_lds_sync();
-        if (!kScatter) {
-            unsigned long long fm = __ballot(dir == kDirFar);
-            if (fm) {
-                int leader = __ffsll((long long)fm) - 1;
-                int fb = 0;
-                if (lane == leader) fb = atomicAdd(&st->far_n[s], __popcll(fm));
-                fb = __shfl(fb, leader, kWave);
-                if (dir == kDirFar) {
-                    int slot = fb + __popcll(fm & lt);
-                    if (slot < far_cap) {
-                        CUPIC_GLOBAL FarRec *f = gp(farl) + slot;
-                        int key = o.njc * G.ncx + o.nic;
-                        f->key = key;
-                        f->idx = i;
-                        f->rank = 0;
-                        f->pad = 0;
-                        f->x = o.x; f->y = o.y; f->vx = o.vx; f->vy = o.vy;
-                        atomicAdd(&far_cnt[key], 1);
-                    } else {
-                        atomicOr(&st->err, kErrFarOverflow);
-                    }
-                }
+        unsigned wx = 0, wy = 0;
+        if (dir < 9) {
+            int dest = rank;  // counters hold slot base + rank
+            if ((unsigned)dest < (unsigned)cap) {
+                st4(POUT, dest, v4f{o.x, o.y, o.vx, o.vy});
+                HOUT[dest] = (unsigned char)push_hint(G, o.fx, o.fy, o.vx, o.vy, o.nic, o.njc);
+            } else {
+                bad = true;
             }
-            probe_n += __popcll(__ballot(dir == kDirProbe));
-            top_n += __popcll(__ballot(dir == kDirTop));
+            fixed_weights(o.fx, o.fy, wx, wy);
+        }
+        const bool stay = (dir == 4);
+        unsigned long long v1 = 0, v2 = 0;
+        if (dir < 9) {
+            v1 = (unsigned long long)wx | ((unsigned long long)wy << 32);
+            v2 = (unsigned long long)wx * (unsigned long long)wy + (1ull << 48);
+        }
+        // stayers: per-lane sums while the wave stays in one old cell,
+        // reduced once per cell (or every 60000 particles); a chunk that
+        // spans cells feeds each cell segment in turn; movers add their own
+        const unsigned long long stay_m = __ballot(stay);
+        if (one_cell) {
+            const int ns = __popcll(stay_m);
+            if (cl0 != acc_cl || acc_n + ns >= 60000) {
+                if (acc_cl >= 0) flush_stayers(tile, TW, acc_cl, acc1, acc2, sgn, lane);
+                acc1 = 0ull;
+                acc2 = 0ull;
+                acc_cl = cl0;
+                acc_n = 0;
+            }
+            if (stay) { acc1 += v1; acc2 += v2; }
+            acc_n += ns;
         } else {
-            unsigned wx = 0, wy = 0;
-            if (dir < 9) {
-                int dest = rank;  // counters hold slot base + rank
-                if ((unsigned)dest < (unsigned)cap) {
-                    st4(POUT, dest, v4f{o.x, o.y, o.vx, o.vy});
-                } else {
-                    bad = true;
-                }
-                fixed_weights(o.fx, o.fy, wx, wy);
-            }
-            const bool stay = (dir == 4);
-            unsigned long long v1 = 0, v2 = 0;
-            if (dir < 9) {
-                v1 = (unsigned long long)wx | ((unsigned long long)wy << 32);
-                v2 = (unsigned long long)wx * (unsigned long long)wy + (1ull << 48);
-            }
-            // stayers: per-lane sums while the wave stays in one old cell,
-            // reduced once per cell (or every 60000 particles); a chunk that
-            // spans cells feeds each cell segment in turn; movers add their own
-            const unsigned long long stay_m = __ballot(stay);
-            if (one_cell) {
-                const int ns = __popcll(stay_m);
-                if (cl0 != acc_cl || acc_n + ns >= 60000) {
+            unsigned long long rem = heads;
+            while (rem) {
+                const int h = __ffsll((long long)rem) - 1;
+                rem &= rem - 1ull;
+                const unsigned long long end =
+                    rem ? ((1ull << (__ffsll((long long)rem) - 1)) - 1ull) : ~0ull;
+                const unsigned long long M = end & ~((1ull << h) - 1ull);
+                const int c = __builtin_amdgcn_readlane(cl, h);
+                if (c < 0 || c >= nc) continue;  // outside the strip: no stayers
+                const int ns = __popcll(stay_m & M);
+                if (c != acc_cl || acc_n + ns >= 60000) {
                     if (acc_cl >= 0) flush_stayers(tile, TW, acc_cl, acc1, acc2, sgn, lane);
                     acc1 = 0ull;
                     acc2 = 0ull;
-                    acc_cl = cl0;
+                    acc_cl = c;
                     acc_n = 0;
                 }
-                if (stay) { acc1 += v1; acc2 += v2; }
+                if (stay && ((M >> lane) & 1ull)) { acc1 += v1; acc2 += v2; }
                 acc_n += ns;
-            } else {
-                unsigned long long rem = heads;
-                while (rem) {
-                    const int h = __ffsll((long long)rem) - 1;
-                    rem &= rem - 1ull;
-                    const unsigned long long end =
-                        rem ? ((1ull << (__ffsll((long long)rem) - 1)) - 1ull) : ~0ull;
-                    const unsigned long long M = end & ~((1ull << h) - 1ull);
-                    const int c = __builtin_amdgcn_readlane(cl, h);
-                    if (c < 0 || c >= nc) continue;  // outside the strip: no stayers
-                    const int ns = __popcll(stay_m & M);
-                    if (c != acc_cl || acc_n + ns >= 60000) {
-                        if (acc_cl >= 0) flush_stayers(tile, TW, acc_cl, acc1, acc2, sgn, lane);
-                        acc1 = 0ull;
-                        acc2 = 0ull;
-                        acc_cl = c;
-                        acc_n = 0;
-                    }
-                    if (stay && ((M >> lane) & 1ull)) { acc1 += v1; acc2 += v2; }
-                    acc_n += ns;
-                }
-            }
-            if (dir < 9 && !stay) {
-                int key = (o.njc - (j0 - 1)) * TW + (cl + o.di + 1);
-                tile_add(tile, TW, key, v1, v2, sgn);
             }
         }
+        if (dir < 9 && !stay) {
+            int key = (o.njc - (j0 - 1)) * TW + (cl + o.di + 1);
+            tile_add(tile, TW, key, v1, v2, sgn);
+        }
     }
-    if (kScatter && acc_cl >= 0) flush_stayers(tile, TW, acc_cl, acc1, acc2, sgn, lane);
+    if (acc_cl >= 0) flush_stayers(tile, TW, acc_cl, acc1, acc2, sgn, lane);
     wave_lds_sync();
-    if (!kScatter) {
-        // stayers = population - movers - departures (probe, top, far)
-        for (int cl = lane; cl < nc; cl += kWave) {
-            int moved = gonew[cl];
-#pragma unroll
-            for (int d = 0; d < 9; d++)
-                if (d != 4) moved += cntw[cl * 9 + d];
-            cntw[cl * 9 + 4] = off[c0 + cl + 1] - off[c0 + cl] - moved;
-        }
-        wave_lds_sync();
-        CUPIC_GLOBAL int *cnt = gp(R.cnt[s]);
-        for (int q = lane; q < nc * 9; q += kWave) {
-            int cl = q / 9, d = q - 9 * cl;
-            cnt[d * ncells + c0 + cl] = cntw[q];
-        }
-        if (lane == 0) {
-            if (probe_n) atomicAdd(&st->probe[s], probe_n);
-            if (top_n) atomicAdd(&st->top[s], top_n);
-        }
-        if (__ballot(bad) && lane == 0) atomicOr(&st->err, kErrSortInvariant);
-    } else {
-        flush_tile(R, tile, TW, j0, col0, lane);
-        if (__ballot(bad) && lane == 0) atomicOr(&st->err, kErrSortInvariant);
-    }
+    flush_tile(R, tile, TW, j0, col0, lane);
+    if (__ballot(bad) && lane == 0) atomicOr(&st->err, kErrSortInvariant);
 }
 
 __global__ __launch_bounds__(kBlock) void k_push_rank(const RunDev *rds, const int *start,
@@ -672,11 +799,14 @@ __global__ __launch_bounds__(kBlock) void k_push_rank(const RunDev *rds, const i
     RUN_PROLOGUE();
     __shared__ int cnts[kBlock / kWave][kCmax * 9];
     __shared__ int gone[kBlock / kWave][kCmax];
+    __shared__ int bnds[kBlock / kWave][kCmax + 1];
+    __shared__ int queues[kBlock / kWave][kQueue];
     __shared__ float2 etiles[kBlock / kWave][2 * (kCmax + 1)];
     TaskInfo T = task_of(R, local);
     if (!T.ok) return;
     int w = threadIdx.x >> 6;
-    push_task<false>(R, st, T, cnts[w], gone[w], etiles[w], nullptr, threadIdx.x & (kWave - 1));
+    count_task(R, st, T, cnts[w], gone[w], bnds[w], queues[w], etiles[w],
+               threadIdx.x & (kWave - 1));
 }
 
 __global__ __launch_bounds__(kBlock) void k_scatter(const RunDev *rds, const int *start,
@@ -689,7 +819,7 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const RunDev *rds, const int
     TaskInfo T = task_of(R, local);
     if (!T.ok) return;
     int w = threadIdx.x >> 6;
-    push_task<true>(R, st, T, cnts[w], nullptr, etiles[w], tiles[w], threadIdx.x & (kWave - 1));
+    scatter_task(R, st, T, cnts[w], etiles[w], tiles[w], threadIdx.x & (kWave - 1));
 }
 
 // ---------------------------------------------------------------- launchers

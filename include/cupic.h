@@ -275,6 +275,21 @@ int cupic_autotune_tile(const cupic_params *p, int32_t steps, int32_t *best_tile
  * the descriptors it was created with: create batches after this call. */
 int cupic_run_set_tile(cupic_run *r, int32_t tile);
 
+/* Push hints of the counting pass (no counterpart in the reference, which
+ * does not sort): the scatter stores with each particle one byte naming the
+ * cell it reaches if its velocity does not change and how far that point
+ * lies from the cell's walls; the next counting pass trusts the byte when
+ * the field cannot move the particle that far, instead of reading and
+ * pushing it.  Modes: CUPIC_HINTS_OFF (push every particle),
+ * CUPIC_HINTS_ON (default), CUPIC_HINTS_CHECKED (push every particle and
+ * flag any trusted hint the push contradicts as a device error).  Results
+ * do not depend on the mode.  A batch keeps the descriptors it was created
+ * with: create batches after this call. */
+#define CUPIC_HINTS_OFF 0
+#define CUPIC_HINTS_ON 1
+#define CUPIC_HINTS_CHECKED 2
+int cupic_run_set_hint_mode(cupic_run *r, int32_t mode);
+
 /* In-situ tuning: steps the run itself `steps` steps per candidate tile
  * (after one step to settle each), from *t (advanced as t += dt per step),
  * and leaves the fastest tile set.  The steps are real steps of the run. */
