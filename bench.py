@@ -431,7 +431,10 @@ def main():
         per = {k: (v[0] / v[1] if v[1] else 0.0) for k, v in kern.items()}
         out["kernels_ms_per_launch"] = per
         dom = "scatter" if per.get("scatter", 0) >= per.get("push_rank", 0) else "push_rank"
-        bytes_per_particle = 32.0 if dom == "scatter" else 16.0
+        # scatter: reads the 16-byte state, writes it and its 1-byte push hint;
+        # push_rank: reads the 1-byte hint (plus the state of the few particles
+        # whose hint the field can overturn, not counted here)
+        bytes_per_particle = 33.0 if dom == "scatter" else 1.0
         dur = per[dom] * 1e-3
         ach = bytes_per_particle * npart / dur if dur > 0 else 0.0
         tpp, src = profiled_traffic("k_" + dom) if a.config == "prod" else (None, None)
@@ -446,11 +449,12 @@ def main():
                              "achieved_GBps": cycle_bytes / (wall / a.steps) / 1e9,
                              "frac": cycle_bytes / (wall / a.steps) / HBM_PEAK,
                              "formula": "84*(Ne+Ni) + nodes*(32 + 12*S)"}
-    design_bytes = 48.0 * npart + nodes * (32.0 + 12.0 * S)
+    design_bytes = 34.0 * npart + nodes * (32.0 + 12.0 * S)
     out["cycle_roofline_design"] = {"bytes_per_step": design_bytes,
                                     "frac": design_bytes / (wall / a.steps) / HBM_PEAK,
-                                    "formula": "48*(Ne+Ni) + nodes*(32 + 12*S): push_rank reads "
-                                               "16 B, scatter reads 16 B and writes 16 B"}
+                                    "formula": "34*(Ne+Ni) + nodes*(32 + 12*S): push_rank reads "
+                                               "the 1 B push hint, scatter reads 16 B and writes "
+                                               "16 B + the 1 B hint"}
     out["probe"] = {"total_absorbed": [int(v) for v in tally["total_probe"]],
                     "charge_collected": tally["charge"]}
     run.close()

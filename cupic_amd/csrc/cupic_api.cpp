@@ -100,6 +100,7 @@ int shape_blocks(const RunDev &h, Shape s)
     switch (s) {
     case kShapeNodes: return (h.nnodes + kBlock - 1) / kBlock;
     case kShapeTasks: return (2 * h.ncy * h.tpr + 3) / 4;
+    case kShapeCountTasks: return (2 * h.ncy * h.tpr1 + 3) / 4;
     case kShapeCellBlk: return 2 * ((h.ncells + kScanCells - 1) / kScanCells);
     case kShapeRunSp: return 2;
     case kShapeRun: return 1;
@@ -278,7 +279,7 @@ void seq_field(const Target &T)
 void seq_mover(const Target &T)
 {
     TimeScope ts(T, kTPushRank);
-    launch_push_rank(T.L(kShapeTasks));
+    launch_push_rank(T.L(kShapeCountTasks));
 }
 
 void seq_cc(const Target &T, int use_host_t, double t, int advance_t, bool clock_forked)
@@ -625,6 +626,8 @@ int cupic_run_create(const cupic_params *pin, int64_t capacity_hint, cupic_run *
     h.nnodes = (int)z.nnodes;
     h.C = z.C;
     h.tpr = (p.ncx + z.C - 1) / z.C;
+    h.C1 = std::min(kCountCells, p.ncx);
+    h.tpr1 = (p.ncx + h.C1 - 1) / h.C1;
     h.cap = (int)z.cap;
     h.far_cap = (int)z.far_cap;
     h.max_inject = (int)z.max_inject;
@@ -1244,7 +1247,7 @@ void batch_step_sequence(cupic_batch *b, const Target &T)
     seq_clock_fork(T, 0, 0.0);
     seq_solve(T, 0, b->mode, 0, 0, b->max_sweeps, b->check);
     launch_field(T.L(kShapeNodes));
-    launch_push_rank(T.L(kShapeTasks));
+    launch_push_rank(T.L(kShapeCountTasks));
     seq_cc(T, 0, 0.0, 1, true);
 }
 

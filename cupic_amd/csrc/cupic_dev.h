@@ -30,6 +30,7 @@ constexpr int kWave = 64;
 constexpr int kBlock = 256;
 constexpr int kCmax = 16;           // max cells per wave task (tile_cells)
 constexpr int kTileW = kCmax + 3;   // deposit tile width (node columns)
+constexpr int kCountCells = 32;     // cells per wave strip of the counting pass (any tile)
 constexpr int kScanCells = 1024;    // cells per scan block
 constexpr int kRedSlots = 16;       // atomic slots per reduced quantity
 constexpr int kFarBlocks = 16;      // far-list kernels: at most this many blocks per species
@@ -117,6 +118,7 @@ enum : int { kHintOff = 0, kHintOn = 1, kHintChecked = 2 };
 struct RunDev {
     int ncx, ncy, ncells, nnodes;
     int C, tpr;                     // cells per wave task, tasks per row
+    int C1, tpr1;                   // the same for the counting pass
     int cap;                        // particles per species per buffer
     int far_cap, max_inject;
     int n_init;                     // initial particles per species

@@ -23,6 +23,7 @@ struct Launch {
 enum Shape : int {
     kShapeNodes = 0,   // ceil(nnodes / 256)
     kShapeTasks,       // ceil(2 * ncy * tpr / 4): one wave per (species, row, column block)
+    kShapeCountTasks,  // ceil(2 * ncy * tpr1 / 4): the same for the counting pass
     kShapeCellBlk,     // 2 * ceil(ncells / kScanCells)
     kShapeRunSp,       // 2: one block per species
     kShapeRun,         // 1

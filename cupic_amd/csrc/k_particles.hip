@@ -207,19 +207,20 @@ struct TaskInfo {
     bool ok;
 };
 
-__device__ __forceinline__ TaskInfo task_of(const RunDev &R, int local)
+// Wave task of workgroup `local` for strips of C cells, tpr strips per row.
+__device__ __forceinline__ TaskInfo task_of(const RunDev &R, int local, int C, int tpr)
 {
     TaskInfo t;
     // the wave index is uniform: keep the task and everything derived from
     // it in scalar registers
     int task = local * (kBlock / kWave) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    int per_species = R.ncy * R.tpr;
+    int per_species = R.ncy * tpr;
     t.ok = task < 2 * per_species;
     t.s = task / per_species;
     int t2 = task - t.s * per_species;
-    t.j0 = t2 / R.tpr;
-    t.col0 = (t2 - t.j0 * R.tpr) * R.C;
-    t.nc = min(R.C, R.ncx - t.col0);
+    t.j0 = t2 / tpr;
+    t.col0 = (t2 - t.j0 * tpr) * C;
+    t.nc = min(C, R.ncx - t.col0);
     return t;
 }
 
@@ -230,7 +231,7 @@ __global__ __launch_bounds__(kBlock) void k_deposit(const RunDev *rds, const int
 {
     RUN_PROLOGUE();
     __shared__ unsigned long long tiles[kBlock / kWave][4 * kTileW];
-    TaskInfo T = task_of(R, local);
+    TaskInfo T = task_of(R, local, R.C, R.tpr);
     if (!T.ok) return;
     int lane = threadIdx.x & (kWave - 1);
     unsigned long long *tile = tiles[threadIdx.x >> 6];
@@ -431,7 +432,8 @@ __device__ __forceinline__ Pushed push_tile(const Geo &R, const float2 *Et, int 
     return o;
 }
 
-constexpr int kQueue = 512;  // pass 1: ring of slots awaiting a full push, per wave
+constexpr int kQueue = 512;   // pass 1: ring of slots awaiting a full push, per wave
+constexpr int kFarList = 64;  // pass 1: far movers listed per wave strip
 
 // Pass 1: one wave per strip of whole old cells counts arrivals per (old
 // cell, direction), appends far movers to the far list and tallies
@@ -443,7 +445,7 @@ constexpr int kQueue = 512;  // pass 1: ring of slots awaiting a full push, per 
 // with its push.
 __device__ __forceinline__ void count_task(const RunDev &R, RunState *st, const TaskInfo &T,
                                            int *cntw, int *gonew, int *bnd, int *queue,
-                                           float2 *Et, int lane)
+                                           int *farq, float2 *Et, int lane)
 {
     const int s = T.s, j0 = T.j0, col0 = T.col0, nc = T.nc;
     const Geo G = geo_of(R);
@@ -487,13 +489,18 @@ __device__ __forceinline__ void count_task(const RunDev &R, RunState *st, const 
     bool bad = false, wrong = false;
 
     // full push of queued slot i (lanes with ok false take no part)
+    // Full push of slot i (lanes with ok false take no part): counts, checks
+    // and the absorption tallies.  Far movers are only listed here (slot
+    // numbers in LDS); their records are written after the loop, which keeps
+    // the loop free of HBM stores and atomics.
+    int nfar = 0;
     auto push_full = [&](int i, bool ok) {
         int dir = kDirNone;
-        Pushed o = {};
+        const v4f p = ld4(PIN, i);
+        const unsigned hh = HIN[i];
         if (ok) {
-            const v4f p = ld4(PIN, i);
             bool in_strip;
-            o = push_tile(G, Et, nc, col0, j0, R.E, qdtm, p.x, p.y, p.z, p.w, in_strip);
+            const Pushed o = push_tile(G, Et, nc, col0, j0, R.E, qdtm, p.x, p.y, p.z, p.w, in_strip);
             dir = o.dir;
             if (dir < 9 && !in_strip) {  // not in this strip: the sort invariant is broken
                 dir = kDirFar;
@@ -504,35 +511,50 @@ __device__ __forceinline__ void count_task(const RunDev &R, RunState *st, const 
                 if (dir < 9) atomicAdd(&cntw[cl * 9 + dir], 1);
                 else atomicAdd(&gonew[cl], 1);
             }
-            if (mode == kHintChecked) {
-                const unsigned h = HIN[i];
-                if ((int)(h >> 4) >= c_need && dir != (int)(h & 15u)) wrong = true;
-            }
+            if (mode == kHintChecked && (int)(hh >> 4) >= c_need && dir != (int)(hh & 15u))
+                wrong = true;
         }
         const unsigned long long fm = __ballot(dir == kDirFar);
-        if (fm) {
-            int leader = __ffsll((long long)fm) - 1;
-            int fb = 0;
-            if (lane == leader) fb = atomicAdd(&st->far_n[s], __popcll(fm));
-            fb = __shfl(fb, leader, kWave);
-            if (dir == kDirFar) {
-                int slot = fb + __popcll(fm & lt);
-                if (slot < far_cap) {
-                    CUPIC_GLOBAL FarRec *f = gp(farl) + slot;
-                    int key = o.njc * G.ncx + o.nic;
-                    f->key = key;
-                    f->idx = i;
-                    f->rank = 0;
-                    f->pad = 0;
-                    f->x = o.x; f->y = o.y; f->vx = o.vx; f->vy = o.vy;
-                    atomicAdd(&far_cnt[key], 1);
-                } else {
-                    atomicOr(&st->err, kErrFarOverflow);
-                }
-            }
+        if (dir == kDirFar) {
+            const int k = nfar + mbcnt64(fm);
+            if (k < kFarList) farq[k] = i;
         }
+        nfar += __popcll(fm);
         probe_n += __popcll(__ballot(dir == kDirProbe));
         top_n += __popcll(__ballot(dir == kDirTop));
+    };
+    // Far record of slot i (the same push again), to the run's far list.
+    auto emit_far = [&](int i, bool ok) {
+        int dir = kDirNone;
+        Pushed o = {};
+        if (ok) {
+            const v4f p = ld4(PIN, i);
+            bool in_strip;
+            o = push_tile(G, Et, nc, col0, j0, R.E, qdtm, p.x, p.y, p.z, p.w, in_strip);
+            dir = o.dir;
+            if (dir < 9 && !in_strip) dir = kDirFar;
+        }
+        const unsigned long long fm = __ballot(dir == kDirFar);
+        if (!fm) return;
+        const int leader = __ffsll((long long)fm) - 1;
+        int fb = 0;
+        if (lane == leader) fb = atomicAdd(&st->far_n[s], __popcll(fm));
+        fb = __shfl(fb, leader, kWave);
+        if (dir == kDirFar) {
+            const int slot = fb + __popcll(fm & lt);
+            if (slot < far_cap) {
+                CUPIC_GLOBAL FarRec *f = gp(farl) + slot;
+                const int key = o.njc * G.ncx + o.nic;
+                f->key = key;
+                f->idx = i;
+                f->rank = 0;
+                f->pad = 0;
+                f->x = o.x; f->y = o.y; f->vx = o.vx; f->vy = o.vy;
+                atomicAdd(&far_cnt[key], 1);
+            } else {
+                atomicOr(&st->err, kErrFarOverflow);
+            }
+        }
     };
 
     // Hints are read as words of four slots, lane l covering slots
@@ -595,7 +617,16 @@ __device__ __forceinline__ void count_task(const RunDev &R, RunState *st, const 
     }
     if (qn > 0) {
         wave_lds_sync();
-        push_full(lane < qn ? queue[(qh + lane) & (kQueue - 1)] : 0, lane < qn);
+        push_full(lane < qn ? queue[(qh + lane) & (kQueue - 1)] : p0u, lane < qn);
+    }
+    // far records: from the list, or, if it overflowed, by pushing the whole
+    // strip again (a strip with more than kFarList far movers is rare)
+    wave_lds_sync();
+    if (nfar <= kFarList) {
+        for (int b0 = 0; b0 < nfar; b0 += kWave)
+            emit_far(b0 + lane < nfar ? farq[b0 + lane] : p0u, b0 + lane < nfar);
+    } else {
+        for (int i0 = p0u; i0 < p1u; i0 += kWave) emit_far(i0 + lane, i0 + lane < p1u);
     }
     wave_lds_sync();
     // stayers = population - movers - departures (probe, top, far)
@@ -797,15 +828,16 @@ __global__ __launch_bounds__(kBlock) void k_push_rank(const RunDev *rds, const i
                                                       int nruns)
 {
     RUN_PROLOGUE();
-    __shared__ int cnts[kBlock / kWave][kCmax * 9];
-    __shared__ int gone[kBlock / kWave][kCmax];
-    __shared__ int bnds[kBlock / kWave][kCmax + 1];
+    __shared__ int cnts[kBlock / kWave][kCountCells * 9];
+    __shared__ int gone[kBlock / kWave][kCountCells];
+    __shared__ int bnds[kBlock / kWave][kCountCells + 1];
     __shared__ int queues[kBlock / kWave][kQueue];
-    __shared__ float2 etiles[kBlock / kWave][2 * (kCmax + 1)];
-    TaskInfo T = task_of(R, local);
+    __shared__ int farqs[kBlock / kWave][kFarList];
+    __shared__ float2 etiles[kBlock / kWave][2 * (kCountCells + 1)];
+    TaskInfo T = task_of(R, local, R.C1, R.tpr1);
     if (!T.ok) return;
     int w = threadIdx.x >> 6;
-    count_task(R, st, T, cnts[w], gone[w], bnds[w], queues[w], etiles[w],
+    count_task(R, st, T, cnts[w], gone[w], bnds[w], queues[w], farqs[w], etiles[w],
                threadIdx.x & (kWave - 1));
 }
 
@@ -816,7 +848,7 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const RunDev *rds, const int
     __shared__ int cnts[kBlock / kWave][kCmax * 9];
     __shared__ float2 etiles[kBlock / kWave][2 * (kCmax + 1)];
     __shared__ unsigned long long tiles[kBlock / kWave][4 * kTileW];
-    TaskInfo T = task_of(R, local);
+    TaskInfo T = task_of(R, local, R.C, R.tpr);
     if (!T.ok) return;
     int w = threadIdx.x >> 6;
     scatter_task(R, st, T, cnts[w], etiles[w], tiles[w], threadIdx.x & (kWave - 1));
