@@ -25,6 +25,7 @@ ERRORS = {-1: "ARG", -2: "HIP", -3: "ALLOC", -4: "CAPACITY", -5: "STATE", -6: "N
 JACOBI_REFERENCE, JACOBI_RESIDUAL, JACOBI_REFINE, POISSON_DIRECT = 0, 1, 2, 3
 HINTS_OFF, HINTS_ON, HINTS_CHECKED = 0, 1, 2
 QUIRK_Q7_REFERENCE_KICK = 1
+QUIRK_Q10_SKIP_FIRST_LINE = 2
 MESH_RHO, MESH_PHI, MESH_EX, MESH_EY, MESH_CHARGE = 0, 1, 2, 3, 4
 ELECTRONS, IONS = 0, 1
 
@@ -109,6 +110,9 @@ def _load():
         "cupic_run_destroy": [vp],
         "cupic_synchronize": [vp],
         "cupic_init_sim": [vp, dp],
+        "cupic_init_sim_restart": [vp, C.c_int32, C.c_char_p, C.c_char_p, dp],
+        "cupic_read_particle_file": [vp, C.c_int, C.c_char_p],
+        "cupic_input_file_set_n_ini": [C.c_char_p, C.c_int32],
         "cupic_create_particles": [vp],
         "cupic_initialize_mesh": [vp],
         "cupic_adjust_leap_frog": [vp],
@@ -206,6 +210,12 @@ def params(**kw):
     return p
 
 
+def input_file_set_n_ini(path, n_ini):
+    """Sets line 2 of the reference's input file to "n_ini = <n_ini>;" (the
+    rewrite at the end of src/main.cu:98-113)."""
+    _check(lib.cupic_input_file_set_n_ini(str(path).encode(), n_ini))
+
+
 def params_from_input_file(path):
     """Reads the reference's input/input_data (src/init.cu:424-789)."""
     p = Params()
@@ -253,6 +263,18 @@ class Run:
         t = C.c_double()
         self._c(lib.cupic_init_sim(self.h, C.byref(t)))
         return t.value
+
+    def init_sim_restart(self, n_ini, ions_path, electrons_path):
+        """Resumes at t = n_ini dt from two particle files (src/init.cu:84-95);
+        returns t."""
+        t = C.c_double()
+        self._c(lib.cupic_init_sim_restart(self.h, n_ini, str(ions_path).encode(),
+                                           str(electrons_path).encode(), C.byref(t)))
+        return t.value
+
+    def read_particle_file(self, species, path):
+        """Loads one species from a file in the snapshot text format."""
+        self._c(lib.cupic_read_particle_file(self.h, species, str(path).encode()))
 
     def create_particles(self):
         self._c(lib.cupic_create_particles(self.h))

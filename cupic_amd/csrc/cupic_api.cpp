@@ -1014,6 +1014,98 @@ int cupic_set_particles(cupic_run *r, int species, const float *x, const float *
     return write_state(r, s);
 }
 
+int cupic_read_particle_file(cupic_run *r, int species, const char *path)
+{
+    if (!r || !path || species < 0 || species > 1)
+        return set_err(nullptr, CUPIC_ERR_ARG, "bad argument");
+    FILE *f = std::fopen(path, "r");
+    if (!f) return set_err(&r->err, CUPIC_ERR_IO, std::string("cannot open ") + path);
+    const bool skip_first = (r->p.quirks & CUPIC_QUIRK_Q10_SKIP_FIRST_LINE) != 0;
+    std::vector<float> x, y, vx, vy;
+    std::string line;
+    long lineno = 0;
+    int st = CUPIC_OK;
+    for (int c = std::fgetc(f); c != EOF || !line.empty(); c = std::fgetc(f)) {
+        if (c != EOF && c != '\n') {
+            line.push_back((char)c);
+            continue;
+        }
+        lineno++;
+        const bool blank = line.find_first_not_of(" \t\r\v\f") == std::string::npos;
+        if (!(skip_first && lineno == 1) && !blank) {
+            double a, b, v, w;
+            char extra;
+            if (std::sscanf(line.c_str(), " %le %le %le %le %c", &a, &b, &v, &w, &extra) != 4) {
+                st = set_err(&r->err, CUPIC_ERR_IO,
+                             std::string(path) + ": line " + std::to_string(lineno) +
+                                 " is not four numbers");
+                break;
+            }
+            x.push_back((float)a);
+            y.push_back((float)b);
+            vx.push_back((float)v);
+            vy.push_back((float)w);
+        }
+        line.clear();
+        if (c == EOF) break;
+    }
+    std::fclose(f);
+    if (st) return st;
+    const float none = 0.0f;
+    auto ptr = [&](std::vector<float> &v) { return v.empty() ? &none : v.data(); };
+    return cupic_set_particles(r, species, ptr(x), ptr(y), ptr(vx), ptr(vy), (int64_t)x.size());
+}
+
+int cupic_init_sim_restart(cupic_run *r, int32_t n_ini, const char *ions_path,
+                           const char *electrons_path, double *t)
+{
+    if (!r || !ions_path || !electrons_path) return set_err(nullptr, CUPIC_ERR_ARG, "bad argument");
+    if (n_ini <= 0) return set_err(&r->err, CUPIC_ERR_ARG, "n_ini must be positive for a restart");
+    // a fresh run state (counts, tallies, clocks), then the two files
+    launch_init_finish(target_of(r).L(kShapeRun));
+    int st = check_launch(r, "init_sim_restart");
+    r->moved = 0;
+    r->acc_valid = false;
+    if (!st) st = cupic_read_particle_file(r, 1, ions_path);   // src/init.cu:338-339
+    if (!st) st = cupic_read_particle_file(r, 0, electrons_path);
+    if (!st) st = cupic_initialize_mesh(r);
+    if (st) return st;
+    const double t0 = (double)n_ini * r->p.dt;  // src/init.cu:86
+    r->t_next = t0 + r->p.dt;
+    if (t) *t = t0;
+    return CUPIC_OK;
+}
+
+int cupic_input_file_set_n_ini(const char *path, int32_t n_ini)
+{
+    if (!path || n_ini < 0) return set_err(nullptr, CUPIC_ERR_ARG, "bad argument");
+    FILE *f = std::fopen(path, "r");
+    if (!f) return set_err(nullptr, CUPIC_ERR_IO, std::string("cannot open ") + path);
+    std::vector<std::string> lines;
+    std::string line;
+    for (int c = std::fgetc(f); c != EOF; c = std::fgetc(f)) {
+        if (c == '\n') {
+            lines.push_back(line);
+            line.clear();
+        } else {
+            line.push_back((char)c);
+        }
+    }
+    if (!line.empty()) lines.push_back(line);
+    std::fclose(f);
+    if (lines.size() < 2) return set_err(nullptr, CUPIC_ERR_IO, std::string(path) + ": no n_ini line");
+    lines[1] = "n_ini = " + std::to_string(n_ini) + ";";  // src/main.cu:104-105
+    const std::string tmp = std::string(path) + ".new";
+    FILE *o = std::fopen(tmp.c_str(), "w");
+    if (!o) return set_err(nullptr, CUPIC_ERR_IO, "cannot write " + tmp);
+    bool ok = true;
+    for (const std::string &l : lines) ok = ok && std::fprintf(o, "%s\n", l.c_str()) >= 0;
+    ok = (std::fclose(o) == 0) && ok;
+    if (!ok || std::rename(tmp.c_str(), path) != 0)
+        return set_err(nullptr, CUPIC_ERR_IO, std::string("cannot replace ") + path);
+    return CUPIC_OK;
+}
+
 int cupic_density_profile(cupic_run *r, int species, int64_t *per_row)
 {
     if (!r || !per_row || species < 0 || species > 1)

@@ -2,13 +2,19 @@
 // reference's src/main.cu:26-117).
 //
 //   cupic_main [--input PATH | --config small|prod] [--steps N] [--out DIR]
-//              [--save-every K] [--jacobi-mode M] [--quiet]
+//              [--save-every K] [--jacobi-mode M] [--restart N]
+//              [--restart-dir DIR] [--quiet]
 //
 // Runs init_sim, then the fixed cycle deposit -> solve -> field -> move ->
 // cc(t) with t accumulated as t += dt (src/main.cu:57-73), writes particle
 // snapshots in the reference's text format every K steps (src/main.cu:79-83)
-// and prints the time, particle counts and the probe tally.  Errors are
-// reported with their status and message; the exit code is non-zero.
+// and prints the time, particle counts and the probe tally.  With n_ini > 0
+// (from the input file, or --restart) the run resumes at t = n_ini dt from
+// DIR/ions.dat and DIR/electrons.dat (src/init.cu:84-95, 338-341; DIR
+// defaults to the working directory, as in the reference).  With --input the
+// file's n_ini line is set to the last step at the end (src/main.cu:98-113).
+// Errors are reported with their status and message; the exit code is
+// non-zero.
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -24,8 +30,8 @@ static int fail(cupic_run *r, int st, const char *what)
 
 int main(int argc, char **argv)
 {
-    std::string input, config = "small", out;
-    int steps = -1, save_every = 0, mode = -1;
+    std::string input, config = "small", out, restart_dir = ".";
+    int steps = -1, save_every = 0, mode = -1, restart = -1;
     bool quiet = false;
     for (int k = 1; k < argc; k++) {
         std::string a = argv[k];
@@ -36,10 +42,13 @@ int main(int argc, char **argv)
         else if (a == "--out") out = next();
         else if (a == "--save-every") save_every = std::atoi(next());
         else if (a == "--jacobi-mode") mode = std::atoi(next());
+        else if (a == "--restart") restart = std::atoi(next());
+        else if (a == "--restart-dir") restart_dir = next();
         else if (a == "--quiet") quiet = true;
         else {
             std::fprintf(stderr, "usage: %s [--input PATH | --config small|prod] [--steps N] "
-                                 "[--out DIR] [--save-every K] [--jacobi-mode M] [--quiet]\n",
+                                 "[--out DIR] [--save-every K] [--jacobi-mode M] [--restart N] "
+                                 "[--restart-dir DIR] [--quiet]\n",
                          argv[0]);
             return 2;
         }
@@ -55,10 +64,6 @@ int main(int argc, char **argv)
     if (!input.empty()) {
         st = cupic_params_from_input_file(input.c_str(), &p, &n_ini, &n_prev, &n_save, &n_fin);
         if (st) return fail(nullptr, st, "params_from_input_file");
-        if (n_ini != 0) {
-            std::fprintf(stderr, "cupic_main: n_ini > 0 (resume from snapshots) is not supported\n");
-            return 2;
-        }
     } else {
         cupic_params_default(&p);
         if (config == "prod") {
@@ -72,17 +77,29 @@ int main(int argc, char **argv)
         if (st) return fail(nullptr, st, "params_derive");
     }
     if (mode >= 0) p.jacobi_mode = mode;
+    if (restart >= 0) n_ini = restart;
     if (steps >= 0) n_fin = steps;
+    if (n_ini < 0) {
+        std::fprintf(stderr, "Wrong input parameter (n_ini<0)\n");  // src/init.cu:97-99
+        return 2;
+    }
     cupic_run *r = nullptr;
     st = cupic_run_create(&p, 0, &r);
     if (st) return fail(nullptr, st, "run_create");
     double t = 0.0;
-    st = cupic_init_sim(r, &t);
-    if (st) return fail(r, st, "init_sim");
     int64_t ne = 0, ni = 0;
-    cupic_number_of_particles(r, 0, &ne);
-    cupic_number_of_particles(r, 1, &ni);
-    std::printf("Simulation initialized with %lld particles.\n", (long long)(ne + ni));
+    if (n_ini == 0) {
+        st = cupic_init_sim(r, &t);
+        if (st) return fail(r, st, "init_sim");
+        cupic_number_of_particles(r, 0, &ne);
+        cupic_number_of_particles(r, 1, &ni);
+        std::printf("Simulation initialized with %lld particles.\n", (long long)(ne + ni));
+    } else {
+        const std::string ions = restart_dir + "/ions.dat", electrons = restart_dir + "/electrons.dat";
+        st = cupic_init_sim_restart(r, n_ini, ions.c_str(), electrons.c_str(), &t);
+        if (st) return fail(r, st, "init_sim_restart");
+        std::printf("Simulation state loaded from time t = %g\n", t);
+    }
     auto snapshot = [&](int i) {
         if (out.empty()) return 0;
         char fn[512];
@@ -117,6 +134,8 @@ int main(int argc, char **argv)
     }
     st = cupic_synchronize(r);
     if (st) return fail(r, st, "synchronize");
+    if (!input.empty() && (st = cupic_input_file_set_n_ini(input.c_str(), n_fin)))
+        return fail(r, st, "input_file_set_n_ini");
     std::printf("Simulation finished!\n");
     cupic_run_destroy(r);
     return 0;

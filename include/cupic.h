@@ -44,6 +44,8 @@ extern "C" {
 
 /* quirk flags */
 #define CUPIC_QUIRK_Q7_REFERENCE_KICK 1u /* injection half-kick sign as committed (src/cc.cu:243-244) */
+#define CUPIC_QUIRK_Q10_SKIP_FIRST_LINE 2u /* particle files: skip the first line as the reference's
+                                               reader does (src/init.cu:372,392), losing a particle */
 
 typedef struct cupic_params {
     int32_t ncx, ncy;        /* cells (nnx = ncx+1, nny = ncy+1 nodes)           */
@@ -116,6 +118,24 @@ int cupic_create_particles(cupic_run *r);
 int cupic_initialize_mesh(cupic_run *r);
 /* adjust_leap_frog (src/init.cu:264-323): v -= dt/2 * qE/m */
 int cupic_adjust_leap_frog(cupic_run *r);
+/* init_sim with n_ini > 0 (src/init.cu:84-95): *t = n_ini dt, both species
+ * from particle files (the reference reads ./ions.dat and ./electrons.dat,
+ * src/init.cu:338-341), then initialize_mesh; no half kick, the files hold
+ * staggered velocities.  Like the reference, the injection clocks start
+ * afresh at the first cc call and the field is solved from the linear
+ * guess. */
+int cupic_init_sim_restart(cupic_run *r, int32_t n_ini, const char *ions_path,
+                           const char *electrons_path, double *t);
+/* read_particle_file (src/init.cu:353-420): one particle per line in the
+ * snapshot format " %.17e %.17e %.17e %.17e " (FP64, rounded to FP32),
+ * uploaded cell-sorted as `species`.  Every line is read unless the run's
+ * quirks hold CUPIC_QUIRK_Q10_SKIP_FIRST_LINE; blank lines are skipped and
+ * any other line that is not four numbers is an error (CUPIC_ERR_IO). */
+int cupic_read_particle_file(cupic_run *r, int species, const char *path);
+/* The end of src/main.cu (98-113): line 2 of the input file becomes
+ * "n_ini = <n_ini>;", the other lines are kept (written to path.new, then
+ * renamed over path). */
+int cupic_input_file_set_n_ini(const char *path, int32_t n_ini);
 
 /* ---- mesh (src/mesh.h:27-29) ----------------------------------------- */
 

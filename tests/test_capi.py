@@ -90,3 +90,25 @@ def test_footprint_scales_with_particles():
     assert 5e6 < small < 1e8
     # 16 B x 2 buffers x 2 species per particle dominates
     assert big > 64 * 104857600 * 0.95
+
+
+def test_input_file_n_ini_rewrite(tmp_path):
+    """The end of src/main.cu (98-113): line 2 becomes "n_ini = n_fin;"; the
+    header and every other line stay as they were."""
+    import cupic_amd as cp
+    txt = ("#execution configuration\nn_ini = 0;\nn_prev = 0;\nn_save = 100;\n"
+           "n_fin = 200000;\nne = 1.0e14;\nTe = 1.0e3;\nbeta = 1.0e-1;\ngamma = 1.0e3;\n"
+           "phi_p = -5.0e0;\nncx = 127;\nncy = 1021;\nds = 1.0e-1;\ndt = 1.0e-3;\n")
+    f = tmp_path / "input_data"
+    f.write_text(txt)
+    cp.input_file_set_n_ini(str(f), 200000)
+    lines = f.read_text().split("\n")
+    want = txt.split("\n")
+    assert lines[1] == "n_ini = 200000;"
+    assert lines[:1] + lines[2:] == want[:1] + want[2:]
+    _, meta = cp.params_from_input_file(str(f))
+    assert meta["n_ini"] == 200000
+    assert not (tmp_path / "input_data.new").exists()
+    with pytest.raises(cp.CupicError) as e:
+        cp.input_file_set_n_ini(str(tmp_path / "missing"), 1)
+    assert e.value.code == -7
