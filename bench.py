@@ -24,6 +24,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+FP64_MFMA_PEAK = 78.6  # MI355X dense FP64 matrix, TFLOP/s
 
 CONFIGS = {
     "prod": dict(ncx=1024, ncy=512, ppc=200, mi=1836.0, kti=0.1, phi_p=-10.0),
@@ -104,9 +105,20 @@ def direct_leg(cp, cfg, steps, warmup, tile):
     n1 = run.number_of_particles(0) + run.number_of_particles(1)
     _, resid = run.last_solve()
     run.close()
+    solve_ms = kern["solve"][0] / max(kern["solve"][1], 1)
+    # the solve's four FP64 products: 2 x (m x ncx x ncx) for the DFT along x
+    # and 2 x (m x m x ncx) for the sine transform along y, m = ncy - 1
+    m, nx = p.ncy - 1, p.ncx
+    flops = 2.0 * (2.0 * m * nx * nx + 2.0 * m * m * nx)
+    achieved = flops / (solve_ms * 1e-3) / 1e12 if solve_ms > 0 else 0.0
     return {"value": 0.5 * (n0 + n1) * steps / wall, "unit": "particle-steps/s",
             "ms_per_step": 1e3 * wall / steps, "poisson_fp64_residual": resid,
-            "solve_ms_per_launch": kern["solve"][0] / max(kern["solve"][1], 1)}
+            "solve_ms_per_launch": solve_ms,
+            "solve_roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_MFMA_PEAK,
+                               "unit": "TFLOP/s", "frac": achieved / FP64_MFMA_PEAK,
+                               "flops_per_solve": flops,
+                               "note": "whole solve (four products, slice sums, FP64 "
+                                       "residual) against the dense FP64 matrix peak"}}
 
 
 def draw_batched(n_runs, seed):

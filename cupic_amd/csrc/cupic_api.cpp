@@ -95,6 +95,16 @@ struct cupic_batch {
 
 namespace cupic {
 
+// Direct solve: output tiles of one product over the m = ncy - 1 interior
+// rows and ncx columns, and the k slices that bring a product near
+// kGSlices workgroups (never more slices than k tiles).
+int dft_tiles(int ncx, int m) { return ((m + kGM - 1) / kGM) * ((ncx + kGN - 1) / kGN); }
+int dft_slices(int ncx, int m, int Kd)
+{
+    const int want = (kGSlices + dft_tiles(ncx, m) - 1) / dft_tiles(ncx, m);
+    return std::max(1, std::min({want, kGMaxSlices, (Kd + kGK - 1) / kGK}));
+}
+
 int shape_blocks(const RunDev &h, Shape s)
 {
     switch (s) {
@@ -114,18 +124,21 @@ int shape_blocks(const RunDev &h, Shape s)
     case kShapeJacBlkN:
         return ((h.ncx + jac_tx(kJacHNarrow) - 1) / jac_tx(kJacHNarrow)) *
                ((h.ncy - 1 + jac_ty(kJacHNarrow) - 1) / jac_ty(kJacHNarrow));
-    case kShapeDftInit:
+    case kShapeDftInit: {
         if (!h.dft_f) return 0;
-        return (int)(((int64_t)h.ncx * 2 * h.nk + kBlock - 1) / kBlock) + (h.nk + kBlock - 1) / kBlock;
-    case kShapeDftFwd:
+        const int64_t m = h.ncy - 1;
+        return (int)(((int64_t)h.ncx * h.ncx + kBlock - 1) / kBlock + (m * m + kBlock - 1) / kBlock +
+                     (h.ncx + m + kBlock - 1) / kBlock);
+    }
+    case kShapeDftGemmX:
         if (!h.dft_f) return 0;
-        return ((h.ncy + 1 + kGM - 1) / kGM) * ((2 * h.nk + kGN - 1) / kGN);
-    case kShapeDftTri:
+        return dft_tiles(h.ncx, h.ncy - 1) * h.nsx;
+    case kShapeDftGemmY:
         if (!h.dft_f) return 0;
-        return (2 * h.nk + kBlock - 1) / kBlock;
-    case kShapeDftInv:
+        return dft_tiles(h.ncx, h.ncy - 1) * h.nsy;
+    case kShapeDftReduce:
         if (!h.dft_f) return 0;
-        return ((h.ncy - 1 + kGM - 1) / kGM) * ((h.ncx + kGN - 1) / kGN);
+        return (int)(((int64_t)(h.ncy - 1) * h.ncx + kBlock - 1) / kBlock);
     default: return 0;
     }
 }
@@ -212,9 +225,10 @@ void seq_solve(const Target &T, int use_args, int mode, double tol, int min_swee
     }
     if (mode == 3) {
         // direct FP64 solve of the same system, then its FP64 residual
-        launch_dft_fwd(T.L(kShapeDftFwd));
-        launch_dft_tri(T.L(kShapeDftTri));
-        launch_dft_inv(T.L(kShapeDftInv));
+        for (int step = 0; step < 4; step++) {
+            launch_dft_gemm(T.L(step == 0 || step == 3 ? kShapeDftGemmX : kShapeDftGemmY), step);
+            launch_dft_reduce(T.L(kShapeDftReduce), step);
+        }
         launch_residual(N, 0, 2, 0);
         launch_solver_end(T.L(kShapeRun), 0, 0);
         return;
@@ -600,8 +614,10 @@ int cupic_run_footprint(const cupic_params *pin, int64_t capacity_hint, int64_t 
     b += z.nnodes * 60;                          // grids
     b += (int64_t)(p.jacobi_max_sweeps + 2) * 12 * kRedSlots;
     if (p.jacobi_mode == 3) {
-        int64_t W = 2 * (p.ncx / 2 + 1);
-        b += 8 * (2 * (int64_t)p.ncx * W + (int64_t)(p.ncy - 1) * (W / 2) + (int64_t)(p.ncy + 1) * W);
+        const int64_t nx = p.ncx, m = p.ncy - 1;
+        const int64_t ns = std::max(dft_slices(p.ncx, p.ncy - 1, p.ncx),
+                                    dft_slices(p.ncx, p.ncy - 1, p.ncy - 1));
+        b += 8 * (2 * nx * nx + m * m + nx + m + (2 + ns) * m * nx);
     }
     *bytes = b;
     return CUPIC_OK;
@@ -699,12 +715,16 @@ int cupic_run_create(const cupic_params *pin, int64_t capacity_hint, cupic_run *
     h.sweep_max = dev_alloc<unsigned int>(r, ((size_t)p.jacobi_max_sweeps + 1) * kRedSlots, &st);
     h.resid = dev_alloc<unsigned long long>(r, ((size_t)p.jacobi_max_sweeps + 2) * kRedSlots, &st);
     if (p.jacobi_mode == 3) {
-        h.nk = p.ncx / 2 + 1;
-        size_t W = 2 * (size_t)h.nk;
-        h.dft_f = dev_alloc<double>(r, (size_t)p.ncx * W, &st);
-        h.dft_g = dev_alloc<double>(r, (size_t)p.ncx * W, &st);
-        h.dft_t = dev_alloc<double>(r, (size_t)(p.ncy - 1) * h.nk, &st);
-        h.dft_z = dev_alloc<double>(r, (size_t)(p.ncy + 1) * W, &st);
+        const size_t nx = (size_t)p.ncx, m = (size_t)p.ncy - 1;
+        h.nsx = dft_slices(p.ncx, p.ncy - 1, p.ncx);
+        h.nsy = dft_slices(p.ncx, p.ncy - 1, p.ncy - 1);
+        h.dft_f = dev_alloc<double>(r, nx * nx, &st);
+        h.dft_g = dev_alloc<double>(r, nx * nx, &st);
+        h.dft_s = dev_alloc<double>(r, m * m, &st);
+        h.dft_d = dev_alloc<double>(r, nx + m, &st);
+        h.dft_z = dev_alloc<double>(r, m * nx, &st);
+        h.dft_y = dev_alloc<double>(r, m * nx, &st);
+        h.dft_p = dev_alloc<double>(r, (size_t)std::max(h.nsx, h.nsy) * m * nx, &st);
     }
     h.st = dev_alloc<RunState>(r, 1, &st);
     r->d_rd = dev_alloc<RunDev>(r, 1, &st);

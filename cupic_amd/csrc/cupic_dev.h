@@ -10,7 +10,8 @@
 //               phi: FP32 ping-pong (modes 0, 1) / FP64 master + FP32
 //               correction ping-pong (mode 2) / FP64 (mode 3, direct);
 //               E: float2 (Ex, Ey).
-//   direct      mode 3 only: DFT bases, Thomas multipliers, transformed rows.
+//   direct      mode 3 only: DFT and sine bases, eigenvalue tables, transformed
+//               rows, per-slice partial sums.
 //   sort meta   cnt[old cell][9 directions], base[new cell][10 slots],
 //               far list of particles that moved more than one cell or were
 //               injected.
@@ -35,8 +36,10 @@ constexpr int kScanCells = 1024;    // cells per scan block
 constexpr int kRedSlots = 16;       // atomic slots per reduced quantity
 constexpr int kFarBlocks = 16;      // far-list kernels: at most this many blocks per species
 constexpr int kGM = 64;             // direct solve: GEMM output tile rows (4 waves x 16)
-constexpr int kGN = 32;             // direct solve: GEMM output tile columns
-constexpr int kGK = 64;             // direct solve: GEMM k tile
+constexpr int kGN = 64;             // direct solve: GEMM output tile columns (4 chains per wave)
+constexpr int kGK = 32;             // direct solve: GEMM k tile
+constexpr int kGSlices = 512;       // direct solve: workgroups a product aims for (k slices)
+constexpr int kGMaxSlices = 8;
 constexpr int kJacRows = 8;         // blocked Jacobi: rows per wave
 constexpr int kJacWaves = 8;        // blocked Jacobi: waves per workgroup
 constexpr int kJacThreads = kJacWaves * 64;
@@ -156,11 +159,13 @@ struct RunDev {
     float2 *E;                      // nnodes
     unsigned int *sweep_max;        // (max_sweeps_alloc + 1) * kRedSlots (float bits)
     unsigned long long *resid;      // (max_sweeps_alloc + 2) * kRedSlots (double bits)
-    int nk;                         // mode 3: wavenumbers 0 .. ncx/2 (K = ncx/2 + 1)
-    double *dft_f;                  // mode 3: [ncx][2K] forward basis
-    double *dft_g;                  // mode 3: [2K][ncx] inverse basis
-    double *dft_t;                  // mode 3: [ncy-1][K] Thomas multipliers
-    double *dft_z;                  // mode 3: [ncy+1][2K] transformed rows
+    int nsx, nsy;                   // mode 3: k slices of the x and y products
+    double *dft_f;                  // mode 3: [ncx][ncx] forward real DFT basis
+    double *dft_g;                  // mode 3: [ncx][ncx] inverse basis
+    double *dft_s;                  // mode 3: [m][m] orthonormal sine basis (m = ncy - 1)
+    double *dft_d;                  // mode 3: [ncx + m] eigenvalue parts per column, per row
+    double *dft_z, *dft_y;          // mode 3: [m][ncx] transformed rows (two stages)
+    double *dft_p;                  // mode 3: [max(nsx, nsy)][m][ncx] partial sums
     RunState *st;
 };
 

@@ -33,10 +33,10 @@ enum Shape : int {
     kShapeInit,        // 2 * ceil(n_init / 256)
     kShapeJacBlk,      // ceil(ncx / jac_tx(kJacH)) * ceil((ncy - 1) / jac_ty(kJacH))
     kShapeJacBlkN,     // the same for the narrow halo kJacHNarrow
-    kShapeDftInit,     // ceil(ncx * 2K / 256) + ceil(K / 256)
-    kShapeDftFwd,      // ceil((ncy + 1) / kGM) * ceil(2K / kGN)
-    kShapeDftTri,      // ceil(2K / 256)
-    kShapeDftInv,      // ceil((ncy - 1) / kGM) * ceil(ncx / kGN)
+    kShapeDftInit,     // ceil(ncx^2 / 256) + ceil(m^2 / 256) + ceil((ncx + m) / 256), m = ncy - 1
+    kShapeDftGemmX,    // ceil(m / kGM) * ceil(ncx / kGN) * nsx
+    kShapeDftGemmY,    // ceil(m / kGM) * ceil(ncx / kGN) * nsy
+    kShapeDftReduce,   // ceil(m * ncx / 256)
     kShapeCount
 };
 
@@ -57,9 +57,8 @@ void launch_field(const Launch &L);
 
 // direct solve (mode 3)
 void launch_dft_init(const Launch &L);
-void launch_dft_fwd(const Launch &L);
-void launch_dft_tri(const Launch &L);
-void launch_dft_inv(const Launch &L);
+void launch_dft_gemm(const Launch &L, int step);
+void launch_dft_reduce(const Launch &L, int step);
 
 // particles
 void launch_create(const Launch &L);
