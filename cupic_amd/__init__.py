@@ -130,12 +130,15 @@ def _load():
         "cupic_set_particles": [vp, C.c_int, vp, vp, vp, vp, C.c_int64],
         "cupic_get_offsets": [vp, C.c_int, vp],
         "cupic_density_profile": [vp, C.c_int, vp],
+        "cupic_get_bookmarks": [vp, C.c_int, vp],
         "cupic_get_mesh": [vp, C.c_int, vp, C.c_int64],
         "cupic_set_phi": [vp, vp, C.c_int64],
         "cupic_get_clock": [vp, dp, C.POINTER(C.c_uint64), i32p],
         "cupic_set_clock": [vp, dp, C.POINTER(C.c_uint64), C.c_int32],
         "cupic_particles_snapshot": [vp, C.c_int, C.c_char_p, C.c_double],
         "cupic_mesh_snapshot": [vp, C.c_int, C.c_char_p],
+        "cupic_bookmarks_snapshot": [vp, C.c_int, C.c_char_p],
+        "cupic_bins_snapshot": [vp, C.c_int, C.c_char_p],
         "cupic_batch_create": [C.POINTER(vp), C.c_int32, C.c_int32, C.POINTER(vp)],
         "cupic_batch_step": [vp, C.c_int32],
         "cupic_autotune_tile": [P, C.c_int32, i32p, dp, dp],
@@ -359,6 +362,12 @@ class Run:
         self._c(lib.cupic_density_profile(self.h, species, a.ctypes.data))
         return a
 
+    def bookmarks(self, species):
+        """row bookmarks (first, last) per cell row, shape (ncy, 2); -1, -1 for an empty row"""
+        a = np.empty(2 * self.p.ncy, np.int32)
+        self._c(lib.cupic_get_bookmarks(self.h, species, a.ctypes.data))
+        return a.reshape(self.p.ncy, 2)
+
     def mesh(self, which):
         """A mesh quantity in the reference layout, shape (nny, nnx)."""
         dt = {MESH_RHO: np.float32, MESH_PHI: np.float64, MESH_EX: np.float32,
@@ -388,6 +397,12 @@ class Run:
 
     def mesh_snapshot(self, which, path):
         self._c(lib.cupic_mesh_snapshot(self.h, which, path.encode()))
+
+    def bookmarks_snapshot(self, species, path):
+        self._c(lib.cupic_bookmarks_snapshot(self.h, species, path.encode()))
+
+    def bins_snapshot(self, species, path):
+        self._c(lib.cupic_bins_snapshot(self.h, species, path.encode()))
 
     def set_tile(self, tile):
         """Cells per wave strip of the push passes (0: from ppc); speed only."""

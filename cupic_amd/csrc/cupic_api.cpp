@@ -1138,6 +1138,24 @@ int cupic_density_profile(cupic_run *r, int species, int64_t *per_row)
     return CUPIC_OK;
 }
 
+// Row bookmarks as the reference keeps them (src/cc.cu:760-763,
+// src/init.cu:401-407): bm[2j] and bm[2j+1] are the first and last index of
+// the particles in cell row j, both -1 for an empty row.  Rows are contiguous
+// in the cell-sorted order, so they follow from the cell offsets.
+int cupic_get_bookmarks(cupic_run *r, int species, int32_t *bm)
+{
+    if (!r || !bm || species < 0 || species > 1) return set_err(nullptr, CUPIC_ERR_ARG, "bad argument");
+    std::vector<int32_t> off((size_t)r->h.ncells + 1);
+    int st = cupic_get_offsets(r, species, off.data());
+    if (st) return st;
+    for (int j = 0; j < r->h.ncy; j++) {
+        int32_t a = off[(size_t)j * r->h.ncx], b = off[(size_t)(j + 1) * r->h.ncx];
+        bm[2 * j] = b > a ? a : -1;
+        bm[2 * j + 1] = b > a ? b - 1 : -1;
+    }
+    return CUPIC_OK;
+}
+
 int cupic_get_offsets(cupic_run *r, int species, int32_t *offsets)
 {
     if (!r || !offsets || species < 0 || species > 1) return set_err(nullptr, CUPIC_ERR_ARG, "bad argument");
@@ -1271,6 +1289,42 @@ int cupic_particles_snapshot(cupic_run *r, int species, const char *path, double
     for (int64_t k = 0; k < n; k++)
         std::fprintf(f, " %.17e %.17e %.17e %.17e \n", (double)x[k], (double)y[k], (double)vx[k],
                      (double)vy[k]);
+    std::fclose(f);
+    return CUPIC_OK;
+}
+
+// save_bm (src/diagnostic.cu:137-167): " %d %d %d \n" per row with the even
+// index 2j first, as the reference's loop prints it.
+int cupic_bookmarks_snapshot(cupic_run *r, int species, const char *path)
+{
+    if (!r || !path) return set_err(nullptr, CUPIC_ERR_ARG, "bad argument");
+    std::vector<int32_t> bm(2 * (size_t)r->h.ncy);
+    int st = cupic_get_bookmarks(r, species, bm.data());
+    if (st) return st;
+    std::string fn = std::string(path) + ".dat";
+    FILE *f = std::fopen(fn.c_str(), "w");
+    if (!f) return set_err(&r->err, CUPIC_ERR_IO, "cannot write " + fn);
+    for (int i = 0; i < 2 * r->h.ncy; i += 2) std::fprintf(f, " %d %d %d \n", i, bm[i], bm[i + 1]);
+    std::fclose(f);
+    return CUPIC_OK;
+}
+
+// save_bins (src/diagnostic.cu:171-215): " %d %d \n" per particle, its index
+// in storage order and int(y/ds) with y widened to double as the reference's
+// float/double division does.
+int cupic_bins_snapshot(cupic_run *r, int species, const char *path)
+{
+    if (!r || !path) return set_err(nullptr, CUPIC_ERR_ARG, "bad argument");
+    int64_t n = 0;
+    int st = cupic_number_of_particles(r, species, &n);
+    if (st) return st;
+    std::vector<float> x(n), y(n), vx(n), vy(n);
+    st = cupic_get_particles(r, species, x.data(), y.data(), vx.data(), vy.data(), n, &n);
+    if (st) return st;
+    std::string fn = std::string(path) + ".dat";
+    FILE *f = std::fopen(fn.c_str(), "w");
+    if (!f) return set_err(&r->err, CUPIC_ERR_IO, "cannot write " + fn);
+    for (int64_t k = 0; k < n; k++) std::fprintf(f, " %d %d \n", (int)k, (int)((double)y[k] / r->h.ds));
     std::fclose(f);
     return CUPIC_OK;
 }

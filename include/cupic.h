@@ -186,6 +186,10 @@ int cupic_get_offsets(cupic_run *r, int species, int32_t *offsets);
  * profile n_s(y) of one species (the reference keeps only the net rho;
  * src/diagnostic.cu:137-215 writes row bookmarks) */
 int cupic_density_profile(cupic_run *r, int species, int64_t *per_row);
+/* row bookmarks in the reference layout, 2*ncy entries: first and last index
+ * of the particles of row j at [2j] and [2j+1], -1 and -1 for an empty row
+ * (the d_e_bm / d_i_bm arrays, src/cc.cu:760-763, src/init.cu:401-407) */
+int cupic_get_bookmarks(cupic_run *r, int species, int32_t *bm);
 
 #define CUPIC_MESH_RHO 0     /* float, charge density                      */
 #define CUPIC_MESH_PHI 1     /* double, potential                          */
@@ -203,6 +207,11 @@ int cupic_set_clock(cupic_run *r, const double tin[2], const uint64_t injected_t
 /* snapshot writers in the reference's text formats (src/diagnostic.cu:48-133) */
 int cupic_particles_snapshot(cupic_run *r, int species, const char *path, double t);
 int cupic_mesh_snapshot(cupic_run *r, int which, const char *path);
+/* debug writers save_bm and save_bins (src/diagnostic.cu:137-215):
+ * " %d %d %d \n" per row (2j, first, last) and " %d %d \n" per particle
+ * (index, int(y/ds)); ".dat" is appended to the path */
+int cupic_bookmarks_snapshot(cupic_run *r, int species, const char *path);
+int cupic_bins_snapshot(cupic_run *r, int species, const char *path);
 
 /* ---- batched runs -------------------------------------------------------- */
 

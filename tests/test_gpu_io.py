@@ -64,6 +64,64 @@ def test_particle_and_mesh_snapshots(dev, tmp_path):
     run.close()
 
 
+def _rows_of(y, ds):
+    # int(y/ds) with the float y widened to double (src/diagnostic.cu:204)
+    return (y.astype(np.float64) / ds).astype(np.int64)
+
+
+@pytest.mark.parametrize("case", ["stepped", "empty_rows"])
+def test_bookmarks_and_bins(dev, tmp_path, case):
+    """save_bm / save_bins (src/diagnostic.cu:137-215): the bookmarks follow
+    from the rows of the downloaded particles, first and last index per row and
+    -1, -1 for an empty row (src/init.cu:401-407), and the files carry the
+    reference's line formats."""
+    p = cp.params(ncx=24, ncy=10, ppc=6)
+    run = cp.Run(p)
+    run.init_sim()
+    if case == "stepped":
+        t = p.dt
+        for _ in range(5):
+            run.step(t)
+            t += p.dt
+    else:
+        # rows 1, 2, 6 and 9 left empty; the rest hold uneven counts
+        rng = np.random.default_rng(7)
+        rows = np.repeat([0, 3, 4, 5, 7, 8], [5, 1, 9, 2, 4, 3])
+        n = rows.size
+        x = (rng.random(n) * p.ncx * p.ds).astype(np.float32)
+        y = ((rows + rng.uniform(0.05, 0.95, n)) * p.ds).astype(np.float32)
+        v = rng.standard_normal((2, n)).astype(np.float32)
+        run.set_particles(0, x, y, v[0], v[1])
+    for s in (0, 1):
+        bm = run.bookmarks(s)
+        y = run.particles(s)[1]
+        rows = _rows_of(y, p.ds)
+        assert np.all(np.diff(rows) >= 0)
+        want = np.full((p.ncy, 2), -1, np.int32)
+        for j in range(p.ncy):
+            idx = np.nonzero(rows == j)[0]
+            if idx.size:
+                want[j] = idx[0], idx[-1]
+        assert np.array_equal(bm, want)
+        if case == "empty_rows" and s == 0:
+            assert np.array_equal(np.nonzero(bm[:, 0] < 0)[0], [1, 2, 6, 9])
+        assert np.array_equal(np.where(bm[:, 0] >= 0, bm[:, 1] - bm[:, 0] + 1, 0),
+                              run.density_profile(s))
+        base = str(tmp_path / f"bm_{s}")
+        run.bookmarks_snapshot(s, base)
+        lines = open(base + ".dat").read().split("\n")
+        assert lines[-1] == "" and len(lines) == p.ncy + 1
+        for j, ln in enumerate(lines[:-1]):
+            assert ln == f" {2 * j} {bm[j, 0]} {bm[j, 1]} "
+        base = str(tmp_path / f"bins_{s}")
+        run.bins_snapshot(s, base)
+        lines = open(base + ".dat").read().split("\n")
+        assert lines[-1] == "" and len(lines) == y.size + 1
+        for k, ln in enumerate(lines[:-1]):
+            assert ln == f" {k} {rows[k]} "
+    run.close()
+
+
 def test_host_driver_matches_binding(dev, tmp_path):
     assert os.path.exists(DRIVER), "cupic_main is built by cupic_amd/build.py"
     say("cupic_main --config small --steps 20")
