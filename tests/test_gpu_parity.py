@@ -72,7 +72,12 @@ def test_half_kick_bit_exact(dev):
 
 
 @pytest.mark.parametrize("kw", [dict(), dict(ncx=37, ncy=9, ppc=7, phi_p=2.0),
-                                dict(quirks=1, mi=4.0), dict(dt=0.2, ppc=5)])
+                                dict(quirks=1, mi=4.0), dict(dt=0.2, ppc=5),
+                                # smallest grid the library takes, the widest
+                                # one, and one with most cells empty
+                                dict(ncx=4, ncy=2, ppc=3), dict(ncx=4096, ncy=2, ppc=2),
+                                dict(ppc=1)],
+                         ids=["base", "odd", "quirks", "long_dt", "min_grid", "max_ncx", "sparse"])
 def test_full_step_bit_exact(dev, kw):
     """deposit -> solve -> field -> move -> cc(t), compared with the FP32
     mirror given the run's own field and clock: sorted states, offsets,
