@@ -205,7 +205,9 @@ __global__ __launch_bounds__(kJacThreads) void k_jacobi_block(const RunDev *rds,
     RUN_PROLOGUE();
     constexpr int NR = kJacRows;
     constexpr int kJacTX = jac_tx(H), kJacTY = jac_ty(H), kJacH = H;
-    __shared__ float xch[2][kJacWaves][2][kWave];
+    // wave-boundary rows of waves -1 .. kJacWaves at index w + 1; the two
+    // outer entries stay zero
+    __shared__ float xch[2][kJacWaves + 2][2][kWave];
     __shared__ double shr[kJacWaves];
     if (st->solve_done) return;
     if (check_slot >= 0) {
@@ -234,6 +236,12 @@ __global__ __launch_bounds__(kJacThreads) void k_jacobi_block(const RunDev *rds,
         out = gp(R.phi32[(st->phi_cur + par + 1) & 1]);
         rhs = gp(R.b32);
     }
+    if (w == 0) {
+        for (int q = 0; q < 2; q++) {
+            xch[q][0][0][lane] = xch[q][0][1][lane] = 0.0f;
+            xch[q][kJacWaves + 1][0][lane] = xch[q][kJacWaves + 1][1][lane] = 0.0f;
+        }
+    }
     float v[NR], b[NR];
     unsigned upd = 0;  // bit i: row i of this thread is updated
     const bool lane_in = lane >= 1 && lane <= kWave - 2;
@@ -251,25 +259,39 @@ __global__ __launch_bounds__(kJacThreads) void k_jacobi_block(const RunDev *rds,
         if (lane_in && lr >= 1 && lr <= NR * kJacWaves - 2 && gy >= 1 && gy < ncy)
             upd |= 1u << i;
     }
+    // Only the Dirichlet rows 0 and ncy, and the rows outside the grid beyond
+    // them, must keep their values.  The region's edge lanes and edge rows
+    // lack a neighbour, so they are wrong after one sweep whether they keep
+    // their value or take the (finite) update, and the valid region shrinks
+    // alike.  Waves without such rows -- all but those of the first and last
+    // tile row -- therefore sweep without a mask.
+    auto sweeps = [&](auto keep) {
 #pragma unroll 1
-    for (int j = 0; j < nsw; j++) {
-        float(*x)[2][kWave] = xch[j & 1];
-        x[w][0][lane] = v[0];
-        x[w][1][lane] = v[NR - 1];
-        __syncthreads();
-        float prev = w > 0 ? x[w - 1][1][lane] : 0.0f;  // old value of the row below
-        const float above = w < kJacWaves - 1 ? x[w + 1][0][lane] : 0.0f;
+        for (int j = 0; j < nsw; j++) {
+            float(*x)[2][kWave] = xch[j & 1];
+            x[w + 1][0][lane] = v[0];
+            x[w + 1][1][lane] = v[NR - 1];
+            __syncthreads();
+            float prev = x[w][1][lane];  // old value of the row below
+            const float above = x[w + 2][0][lane];
 #pragma unroll
-        for (int i = 0; i < NR; i++) {
-            const float cur = v[i];
-            float W = lane_prev(cur);
-            float E = lane_next(cur);
-            float N = i < NR - 1 ? v[i + 1] : above;
-            float nvv = 0.25f * ((((b[i] + W) + E) + N) + prev);
-            prev = cur;
-            v[i] = (upd >> i) & 1u ? nvv : cur;
+            for (int i = 0; i < NR; i++) {
+                const float cur = v[i];
+                float W = lane_prev(cur);
+                float E = lane_next(cur);
+                float N = i < NR - 1 ? v[i + 1] : above;
+                float nvv = 0.25f * ((((b[i] + W) + E) + N) + prev);
+                prev = cur;
+                v[i] = keep(i) ? cur : nvv;
+            }
         }
-    }
+    };
+    const bool dirichlet_wave =
+        __builtin_amdgcn_readfirstlane((row0 <= 0 || row0 + NR - 1 >= ncy) ? 1 : 0) != 0;
+    if (dirichlet_wave)
+        sweeps([&](int i) { return ((upd >> i) & 1u) == 0u; });
+    else
+        sweeps([](int) { return false; });
     const bool col_in = lane >= kJacH && lane < kJacH + kJacTX && bx * kJacTX + lane - kJacH < ncx;
 #pragma unroll
     for (int i = 0; i < NR; i++) {
@@ -280,11 +302,11 @@ __global__ __launch_bounds__(kJacThreads) void k_jacobi_block(const RunDev *rds,
     if (resid_slot < 0) return;
     // FP64 residual of the tile's interior nodes after the last sweep
     float(*x)[2][kWave] = xch[nsw & 1];
-    x[w][0][lane] = v[0];
-    x[w][1][lane] = v[NR - 1];
+    x[w + 1][0][lane] = v[0];
+    x[w + 1][1][lane] = v[NR - 1];
     __syncthreads();
-    const float below = w > 0 ? x[w - 1][1][lane] : 0.0f;
-    const float above = w < kJacWaves - 1 ? x[w + 1][0][lane] : 0.0f;
+    const float below = x[w][1][lane];
+    const float above = x[w + 2][0][lane];
     const CUPIC_GLOBAL double *b64 = gp(R.b64);
     double rmax = 0.0;
 #pragma unroll
