@@ -230,6 +230,17 @@ __device__ __forceinline__ int find_run(const int *__restrict__ start, int nruns
     return lo;
 }
 
+// Prologue of every kernel that takes (rds, start, nruns): the run R of this
+// workgroup, its block index `local` within the run, and the run state st.
+// start[0] is 0, so a launch for one run needs no load from start.
+#define RUN_PROLOGUE()                                                  \
+    const int b_ = blockIdx.x;                                          \
+    const int r_ = nruns == 1 ? 0 : find_run(start, nruns, b_);         \
+    const RunDev &R = rds[r_];                                          \
+    const int local = nruns == 1 ? b_ : b_ - start[r_];                 \
+    RunState *st = R.st;                                                \
+    (void)local; (void)st;
+
 __device__ __forceinline__ void wave_lds_sync()
 {
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");

@@ -8,13 +8,6 @@
 
 namespace cupic {
 
-#define RUN_PROLOGUE()                                              \
-    const int b_ = blockIdx.x;                                      \
-    const int r_ = find_run(start, nruns, b_);                      \
-    const RunDev &R = rds[r_];                                      \
-    const int local = b_ - start[r_];                               \
-    RunState *st = R.st;                                            \
-    (void)local; (void)st;
 
 constexpr double kPi = 3.1415926535897932;  // src/stdh.h:26
 

@@ -32,13 +32,6 @@
 
 namespace cupic {
 
-#define RUN_PROLOGUE()                                              \
-    const int b_ = blockIdx.x;                                      \
-    const int r_ = find_run(start, nruns, b_);                      \
-    const RunDev &R = rds[r_];                                      \
-    const int local = b_ - start[r_];                               \
-    RunState *st = R.st;                                            \
-    (void)local; (void)st;
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 

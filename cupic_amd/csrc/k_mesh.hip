@@ -6,13 +6,6 @@
 
 namespace cupic {
 
-#define RUN_PROLOGUE()                                              \
-    const int b_ = blockIdx.x;                                      \
-    const int r_ = find_run(start, nruns, b_);                      \
-    const RunDev &R = rds[r_];                                      \
-    const int local = b_ - start[r_];                               \
-    RunState *st = R.st;                                            \
-    (void)local; (void)st;
 
 // src/init.cu:235-241: phi(j) = (1 - j/ncy) phi_p in every phi buffer.
 __global__ __launch_bounds__(kBlock) void k_phi_linear(const RunDev *rds, const int *start,
@@ -209,15 +202,17 @@ __global__ __launch_bounds__(kJacThreads) void k_jacobi_block(const RunDev *rds,
     // outer entries stay zero
     __shared__ float xch[2][kJacWaves + 2][2][kWave];
     __shared__ double shr[kJacWaves];
-    if (st->solve_done) return;
-    if (check_slot >= 0) {
-        double res = read_resid(R, check_slot);
-        if (res <= st->solve_tol || s0 >= st->solve_max) {
-            mark_done(R, st, local, s0, res, par);
-            return;
-        }
+    // the run state and the residual slots in one round trip, ahead of the
+    // exits that depend on them
+    const int done = st->solve_done, phi_cur = st->phi_cur, smax = st->solve_max;
+    const double tol = st->solve_tol;
+    const double res = check_slot >= 0 ? read_resid(R, check_slot) : 0.0;
+    if (done) return;
+    if (check_slot >= 0 && (res <= tol || s0 >= smax)) {
+        mark_done(R, st, local, s0, res, par);
+        return;
     }
-    if (s0 >= st->solve_max) return;
+    if (s0 >= smax) return;
     const int ncx = R.ncx, ncy = R.ncy;
     const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x >> 6;
     const int tiles_x = (ncx + kJacTX - 1) / kJacTX;
@@ -232,8 +227,8 @@ __global__ __launch_bounds__(kJacThreads) void k_jacobi_block(const RunDev *rds,
         out = gp(R.d32[(par + 1) & 1]);
         rhs = gp(R.r32);
     } else {
-        in = gp(R.phi32[(st->phi_cur + par) & 1]);
-        out = gp(R.phi32[(st->phi_cur + par + 1) & 1]);
+        in = gp(R.phi32[(phi_cur + par) & 1]);
+        out = gp(R.phi32[(phi_cur + par + 1) & 1]);
         rhs = gp(R.b32);
     }
     if (w == 0) {
