@@ -25,6 +25,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
 FP64_MFMA_PEAK = 78.6  # MI355X dense FP64 matrix, TFLOP/s
+BREAKDOWN_STEPS = 3    # steps after the timed ones for the per-class kernel times
 
 CONFIGS = {
     "prod": dict(ncx=1024, ncy=512, ppc=200, mi=1836.0, kti=0.1, phi_p=-10.0),
@@ -93,7 +94,7 @@ def direct_leg(cp, cfg, steps, warmup, tile):
         t += p.dt
     run.synchronize()
     n0 = run.number_of_particles(0) + run.number_of_particles(1)
-    run.timing(True)
+    run.timing(True, classes=["solve"])
     run.timing_read()
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -399,8 +400,12 @@ def main():
     print("# warm-up done", file=sys.stderr, flush=True)
     n0 = run.number_of_particles(0) + run.number_of_particles(1)
     sw0, _ = run.solver_totals()
+    # The timed steps carry HIP events around the scatter only (the kernel
+    # the roofline is quoted for); events around every kernel class cost
+    # about 3 % of a step, so the per-class breakdown comes from a few more
+    # steps after the timed ones.
     if not a.no_timing:
-        run.timing(True)
+        run.timing(True, classes=["scatter"])
         run.timing_read()
     run.synchronize()
     t0 = time.perf_counter()
@@ -411,7 +416,17 @@ def main():
     wall = time.perf_counter() - t0
     n1 = run.number_of_particles(0) + run.number_of_particles(1)
     sw1, _ = run.solver_totals()
-    kern = run.timing_read() if not a.no_timing else {}
+    kern = {}
+    if not a.no_timing:
+        kern_timed = run.timing_read()
+        run.timing(True)
+        run.timing_read()
+        for _ in range(BREAKDOWN_STEPS):
+            run.step(t)
+            t += p.dt
+        run.synchronize()
+        kern = run.timing_read()
+        kern["scatter"] = kern_timed["scatter"]
     run.timing(False)
     ps = 0.5 * (n0 + n1) * a.steps
     value = ps / wall
@@ -442,11 +457,14 @@ def main():
     if kern:
         per = {k: (v[0] / v[1] if v[1] else 0.0) for k, v in kern.items()}
         out["kernels_ms_per_launch"] = per
-        dom = "scatter" if per.get("scatter", 0) >= per.get("push_rank", 0) else "push_rank"
-        # scatter: reads the 16-byte state, writes it and its 1-byte push hint;
-        # push_rank: reads the 1-byte hint (plus the state of the few particles
-        # whose hint the field can overturn, not counted here)
-        bytes_per_particle = 33.0 if dom == "scatter" else 1.0
+        out["kernels_ms_per_launch_source"] = (
+            f"scatter: HIP events around each k_scatter of the {a.steps} timed steps; "
+            f"the other classes: {BREAKDOWN_STEPS} further steps with events around "
+            f"every kernel class")
+        # the dominant kernel: the scatter reads the 16-byte state and writes
+        # it and its 1-byte push hint
+        dom = "scatter"
+        bytes_per_particle = 33.0
         dur = per[dom] * 1e-3
         ach = bytes_per_particle * npart / dur if dur > 0 else 0.0
         tpp, src = profiled_traffic("k_" + dom) if a.config == "prod" else (None, None)

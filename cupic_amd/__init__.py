@@ -157,6 +157,7 @@ def _load():
         "cupic_server_destroy": [vp],
         "cupic_batch_destroy": [vp],
         "cupic_timing_enable": [vp, C.c_int32],
+        "cupic_timing_select": [vp, C.c_uint32],
         "cupic_timing_read": [vp, dp, i64p, C.c_int32],
     }
     for name, args in sig.items():
@@ -426,8 +427,17 @@ class Run:
         self.p.tile_cells = best.value
         return tt.value, best.value, ms.value, {c: times[c] for c in range(17) if times[c] > 0}
 
-    def timing(self, on=True):
-        self._c(lib.cupic_timing_enable(self.h, 1 if on else 0))
+    def timing(self, on=True, classes=None):
+        """Per-class device time with HIP events; `classes` (names from
+        timing_read) limits the events to those classes."""
+        if on and classes:
+            names = lib.cupic_timing_names().decode().split(",")
+            mask = 0
+            for c in classes:
+                mask |= 1 << names.index(c)
+            self._c(lib.cupic_timing_select(self.h, mask))
+        else:
+            self._c(lib.cupic_timing_enable(self.h, 1 if on else 0))
 
     def timing_read(self):
         names = lib.cupic_timing_names().decode().split(",")

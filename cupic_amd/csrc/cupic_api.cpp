@@ -71,6 +71,7 @@ struct cupic_run {
     double t_next = 0.0;
     int64_t footprint = 0;
     bool timing = false;
+    unsigned t_mask = (1u << kTClasses) - 1;  // classes timed while timing is on
     struct Ev { int cls; hipEvent_t a, b; };
     std::vector<Ev> evs;
     std::vector<hipEvent_t> pool;
@@ -191,7 +192,8 @@ struct TimeScope {
     cupic_run *r;
     int cls;
     hipEvent_t a = nullptr, b = nullptr;
-    TimeScope(const Target &T, int c) : r(T.timed), cls(c)
+    TimeScope(const Target &T, int c)
+        : r(T.timed && ((T.timed->t_mask >> c) & 1u) ? T.timed : nullptr), cls(c)
     {
         if (!r) return;
         a = take(); b = take();
@@ -1363,6 +1365,17 @@ int cupic_timing_enable(cupic_run *r, int32_t on)
 {
     if (!r) return set_err(nullptr, CUPIC_ERR_ARG, "null run");
     r->timing = on != 0;
+    r->t_mask = (1u << kTClasses) - 1;
+    return CUPIC_OK;
+}
+
+int cupic_timing_select(cupic_run *r, uint32_t class_mask)
+{
+    if (!r) return set_err(nullptr, CUPIC_ERR_ARG, "null run");
+    if (class_mask == 0 || (class_mask >> kTClasses) != 0)
+        return set_err(&r->err, CUPIC_ERR_ARG, "class mask must select classes 0..7");
+    r->timing = true;
+    r->t_mask = class_mask;
     return CUPIC_OK;
 }
 

@@ -330,6 +330,10 @@ int cupic_autotune_run(cupic_run *r, double *t, int32_t steps, int32_t *best_til
 /* Per-kernel-class device time (ms) accumulated with HIP events on the run's
  * stream while timing is on; classes are listed by cupic_timing_names. */
 int cupic_timing_enable(cupic_run *r, int32_t on);
+/* Turns timing on for the classes in class_mask only (bit k: class k of
+ * cupic_timing_names), so that a timed run carries the events of the kernels
+ * it reports and no others; cupic_timing_enable(r, 1) times every class. */
+int cupic_timing_select(cupic_run *r, uint32_t class_mask);
 int cupic_timing_read(cupic_run *r, double *ms, int64_t *launches, int32_t n);
 const char *cupic_timing_names(void);
 
