@@ -339,8 +339,11 @@ __device__ __forceinline__ void cell_of(const G &R, float x, float y, float &fx,
     fx = tx - bx;
     fy = ty - by;
     // inside the grid the clamps below change nothing; outside (or NaN) they
-    // pin the cell to the edge and the fractions into [0, 1]
-    if (__builtin_expect(!(tx >= 0.0f && tx < (float)R.ncx && ty >= 0.0f && ty < (float)R.ncy), 0)) {
+    // pin the cell to the edge and the fractions into [0, 1].  The wave
+    // branches around them (a ballot) so that they cost nothing when no lane
+    // needs them, which is almost always.
+    const bool out = !(tx >= 0.0f && tx < (float)R.ncx && ty >= 0.0f && ty < (float)R.ncy);
+    if (__builtin_expect(__ballot(out) != 0ull, 0) && out) {
         bx = fminf(fmaxf(bx, 0.0f), (float)(R.ncx - 1));
         by = fminf(fmaxf(by, 0.0f), (float)(R.ncy - 1));
         fx = fminf(fmaxf(tx - bx, 0.0f), 1.0f);
