@@ -242,21 +242,33 @@ void seq_solve(const Target &T, int use_args, int mode, double tol, int min_swee
     const int per = halo - 1;  // sweeps per blocked launch
     int c = 0, s = 0, nlaunch = 0, dpar = 0;
     if (mode == 1) {
-        // FP64 residuals: the initial one here, the others fused into the
-        // last launch of each block of check sweeps
+        // FP64 residual checks at sweeps 0, check, 2 check, ... and
+        // max_sweeps, check k in slot k: the first here, the others reduced
+        // by the launch whose sweeps reach them.  Launches of `per` sweeps
+        // run straight through the check points; a check inside a launch
+        // keeps its iterate in the third phi buffer, and the next launch
+        // reads the check and ends the solve there if it is met.
         launch_residual(N, 0, mode, 0);
+        const int step = std::min(per, check);  // so at most one check per launch
+        int cslot = 0, cat = 0, cmid = 0;       // the check the next launch reads
         while (s < max_sweeps) {
-            int k = std::min(check, max_sweeps - s);
-            for (int q = 0; q < k;) {
-                int nsw = std::min(per, k - q);
-                bool last = q + nsw >= k;
-                launch_jacobi_block(B, halo, s + q, nsw, 1, q == 0 ? c : -1, 0, nlaunch & 1,
-                                    last ? c + 1 : -1);
-                nlaunch++;
-                q += nsw;
+            const int nsw = std::min(step, max_sweeps - s);
+            const int next_at = std::min((c + 1) * check, max_sweeps);
+            int rslot = -1, rat = 0;
+            if (next_at <= s + nsw) {
+                rslot = c + 1;
+                rat = next_at - s;
             }
-            s += k;
-            c += 1;
+            launch_jacobi_block(B, halo, s, nsw, 1, 0, nlaunch & 1, cslot, cat, cmid, rslot, rat);
+            nlaunch++;
+            cslot = -1;
+            if (rslot >= 0) {
+                c += 1;
+                cslot = c;
+                cat = next_at;
+                cmid = rat < nsw ? 1 : 0;
+            }
+            s += nsw;
         }
         launch_solver_end(T.L(kShapeRun), c, nlaunch & 1);
         return;
@@ -269,7 +281,8 @@ void seq_solve(const Target &T, int use_args, int mode, double tol, int min_swee
         int dl = 0;
         for (int q = 0; q < k;) {
             int nsw = std::min(per, k - q);
-            launch_jacobi_block(B, halo, s + q, nsw, 2, q == 0 ? c : -1, q == 0 ? 1 : 0, dl & 1, -1);
+            launch_jacobi_block(B, halo, s + q, nsw, 2, q == 0 ? 1 : 0, dl & 1, q == 0 ? c : -1,
+                                s + q, 0, -1, 0);
             dl++;
             q += nsw;
         }
@@ -613,7 +626,7 @@ int cupic_run_footprint(const cupic_params *pin, int64_t capacity_hint, int64_t 
     b += 2 * 2 * (z.ncells + 1) * 4;             // offsets
     b += 2 * z.ncells * (9 + 10 + 1 + 1 + 1) * 4;  // cnt, base, far_cnt, far_cur, tot
     b += 2 * z.far_cap * (int64_t)sizeof(FarRec) + 2 * z.max_inject * 8;
-    b += z.nnodes * 60;                          // grids
+    b += z.nnodes * 64;                          // grids
     b += (int64_t)(p.jacobi_max_sweeps + 2) * 12 * kRedSlots;
     if (p.jacobi_mode == 3) {
         const int64_t nx = p.ncx, m = p.ncy - 1;
@@ -709,6 +722,7 @@ int cupic_run_create(const cupic_params *pin, int64_t capacity_hint, cupic_run *
     h.b64 = dev_alloc<double>(r, z.nnodes, &st);
     h.phi32[0] = dev_alloc<float>(r, z.nnodes, &st);
     h.phi32[1] = dev_alloc<float>(r, z.nnodes, &st);
+    h.phi32[2] = dev_alloc<float>(r, z.nnodes, &st);
     h.phi64 = dev_alloc<double>(r, z.nnodes, &st);
     h.d32[0] = dev_alloc<float>(r, z.nnodes, &st);
     h.d32[1] = dev_alloc<float>(r, z.nnodes, &st);
@@ -1238,8 +1252,8 @@ int cupic_set_phi(cupic_run *r, const double *phi, int64_t count)
         }
     HIPTRY(&r->err, hipStreamSynchronize(r->stream));
     HIPTRY(&r->err, hipMemcpy(h.phi64, v.data(), nn * 8, hipMemcpyHostToDevice));
-    HIPTRY(&r->err, hipMemcpy(h.phi32[0], f.data(), nn * 4, hipMemcpyHostToDevice));
-    HIPTRY(&r->err, hipMemcpy(h.phi32[1], f.data(), nn * 4, hipMemcpyHostToDevice));
+    for (int q = 0; q < 3; q++)
+        HIPTRY(&r->err, hipMemcpy(h.phi32[q], f.data(), nn * 4, hipMemcpyHostToDevice));
     return CUPIC_OK;
 }
 

@@ -152,7 +152,7 @@ struct RunDev {
     float *rho;                     // nnodes
     float *b32;                     // nnodes, Jacobi rhs
     double *b64;                    // nnodes
-    float *phi32[2];                // nnodes, modes 0/1
+    float *phi32[3];                // nnodes, modes 0/1 (see phi_at)
     double *phi64;                  // nnodes, mode 2 master
     float *d32[2];                  // nnodes, mode 2 correction
     float *r32;                     // nnodes, mode 2 correction rhs
@@ -229,6 +229,15 @@ __device__ __forceinline__ int find_run(const int *__restrict__ start, int nruns
     }
     return lo;
 }
+
+// FP32 potential buffers of modes 0 and 1.  A solve starts from buffer
+// a = phi_cur and alternates between a and phi_alt(a): after `par` sweeps
+// (mode 0) or blocked launches (mode 1) the iterate is in phi_at(a, par).
+// The third buffer, phi_third(a), receives the iterate of a residual check
+// that falls inside a blocked launch, in case the solve stops there.
+__host__ __device__ inline int phi_alt(int a) { return a == 0 ? 1 : 0; }
+__host__ __device__ inline int phi_at(int a, int par) { return (par & 1) ? phi_alt(a) : a; }
+__host__ __device__ inline int phi_third(int a) { return 3 - a - phi_alt(a); }
 
 // Prologue of every kernel that takes (rds, start, nruns): the run R of this
 // workgroup, its block index `local` within the run, and the run state st.

@@ -48,8 +48,8 @@ void launch_rho_convert(const Launch &L);
 void launch_solver_begin(const Launch &L, int use_args, int mode, double tol, int min_sweeps,
                          int max_sweeps, int check_every);
 void launch_jacobi_sweep(const Launch &L, int s);
-void launch_jacobi_block(const Launch &L, int halo, int s0, int nsw, int mode, int check_slot,
-                         int first, int par, int resid_slot);
+void launch_jacobi_block(const Launch &L, int halo, int s0, int nsw, int mode, int first,
+                         int par, int cslot, int cat, int cmid, int rslot, int rat);
 void launch_ir_update(const Launch &L, int par);
 void launch_residual(const Launch &L, int c, int mode, int par);
 void launch_solver_end(const Launch &L, int last_check, int par);

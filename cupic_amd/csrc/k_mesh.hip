@@ -18,6 +18,7 @@ __global__ __launch_bounds__(kBlock) void k_phi_linear(const RunDev *rds, const 
     double v = (1.0 - (double)j / (double)R.ncy) * R.phi_p;
     R.phi32[0][k] = (float)v;
     R.phi32[1][k] = (float)v;
+    R.phi32[2][k] = (float)v;
     R.phi64[k] = v;
     R.d32[0][k] = 0.0f;
     R.d32[1][k] = 0.0f;
@@ -89,13 +90,16 @@ __device__ __forceinline__ float read_sweep_max(const RunDev &R, int s)
     return m;
 }
 
+// Ends a solve at sweep s with residual res; the solution is in phi buffer
+// phi_at(phi_cur, par), or in the third buffer when `third` (a check inside
+// a blocked launch).
 __device__ __forceinline__ void mark_done(const RunDev &R, RunState *st, int local, int s,
-                                          double res, int par)
+                                          double res, int par, bool third = false)
 {
     if (local == 0 && threadIdx.x == 0) {
         st->sweeps_done = s;
         st->last_resid = res;
-        st->phi_final = (st->phi_cur + par) & 1;
+        st->phi_final = third ? phi_third(st->phi_cur) : phi_at(st->phi_cur, par);
         st->solve_done = 1;
     }
 }
@@ -147,8 +151,8 @@ __global__ __launch_bounds__(kBlock) void k_jacobi_sweep(const RunDev *rds, cons
             int kw = (i == 0) ? k + ncx - 1 : k - 1;
             int ke = (i == ncx - 1) ? k - ncx + 1 : k + 1;
             int pc = st->phi_cur;
-            const CUPIC_GLOBAL float *in = gp(R.phi32[(pc + s) & 1]);
-            CUPIC_GLOBAL float *out = gp(R.phi32[(pc + s + 1) & 1]);
+            const CUPIC_GLOBAL float *in = gp(R.phi32[phi_at(pc, s)]);
+            CUPIC_GLOBAL float *out = gp(R.phi32[phi_at(pc, s + 1)]);
             float c = in[k];
             float v = 0.25f * ((((gp(R.b32)[k] + in[kw]) + in[ke]) + in[k + ncx]) + in[k - ncx]);
             out[k] = v;
@@ -186,14 +190,19 @@ __device__ __forceinline__ float lane_next(float v)
 // sweep and still covers the tile and one ring after nsw sweeps, so the
 // iterates equal those of nsw single sweeps (same operations per node).
 // Mode 1 relaxes phi (rhs b), mode 2 the refinement correction d (rhs r),
-// d starting at 0 when first.  par selects the input buffer; the output goes
-// to the other one.  With resid_slot >= 0 (mode 1) the launch also reduces
-// the FP64 residual of its tile after the last sweep.
+// d starting at 0 when first.  par counts the launches of the solve so far
+// and selects the input buffer; the output goes to the other one.
+// Residual checks (mode 1): with cslot >= 0 the launch first reads the FP64
+// residual of the check at sweep cat and ends the solve if it is met; that
+// iterate is the launch's input, or the third phi buffer when the check fell
+// inside the previous launch (cmid).  With rslot >= 0 the launch reduces the
+// FP64 residual of its tile after rat of its sweeps into slot rslot; when
+// rat < nsw it also stores the tile at that sweep into the third buffer.
 template <int H>
 __global__ __launch_bounds__(kJacThreads) void k_jacobi_block(const RunDev *rds, const int *start,
                                                          int nruns, int s0, int nsw, int mode,
-                                                         int check_slot, int first, int par,
-                                                         int resid_slot)
+                                                         int first, int par, int cslot, int cat,
+                                                         int cmid, int rslot, int rat)
 {
     RUN_PROLOGUE();
     constexpr int NR = kJacRows;
@@ -206,10 +215,10 @@ __global__ __launch_bounds__(kJacThreads) void k_jacobi_block(const RunDev *rds,
     // exits that depend on them
     const int done = st->solve_done, phi_cur = st->phi_cur, smax = st->solve_max;
     const double tol = st->solve_tol;
-    const double res = check_slot >= 0 ? read_resid(R, check_slot) : 0.0;
+    const double res = cslot >= 0 ? read_resid(R, cslot) : 0.0;
     if (done) return;
-    if (check_slot >= 0 && (res <= tol || s0 >= smax)) {
-        mark_done(R, st, local, s0, res, par);
+    if (cslot >= 0 && (res <= tol || cat >= smax)) {
+        mark_done(R, st, local, cat, res, par, cmid != 0);
         return;
     }
     if (s0 >= smax) return;
@@ -227,8 +236,8 @@ __global__ __launch_bounds__(kJacThreads) void k_jacobi_block(const RunDev *rds,
         out = gp(R.d32[(par + 1) & 1]);
         rhs = gp(R.r32);
     } else {
-        in = gp(R.phi32[(phi_cur + par) & 1]);
-        out = gp(R.phi32[(phi_cur + par + 1) & 1]);
+        in = gp(R.phi32[phi_at(phi_cur, par)]);
+        out = gp(R.phi32[phi_at(phi_cur, par + 1)]);
         rhs = gp(R.b32);
     }
     if (w == 0) {
@@ -254,15 +263,58 @@ __global__ __launch_bounds__(kJacThreads) void k_jacobi_block(const RunDev *rds,
         if (lane_in && lr >= 1 && lr <= NR * kJacWaves - 2 && gy >= 1 && gy < ncy)
             upd |= 1u << i;
     }
+    const bool col_in = lane >= kJacH && lane < kJacH + kJacTX && bx * kJacTX + lane - kJacH < ncx;
+    // the tile's nodes in a grid buffer
+    auto store_tile = [&](CUPIC_GLOBAL float *dst) {
+#pragma unroll
+        for (int i = 0; i < NR; i++) {
+            int lr = NR * w + i, gy = row0 + i;
+            if (col_in && lr >= kJacH && lr < kJacH + kJacTY && gy >= 1 && gy < ncy)
+                dst[gy * ncx + (bx * kJacTX + lane - kJacH)] = v[i];
+        }
+    };
+    // the FP64 right-hand side of the tile's nodes (0 elsewhere), loaded a
+    // sweep ahead of the residual that needs it
+    double b64t[NR];
+    auto load_b64 = [&]() {
+        const CUPIC_GLOBAL double *b64 = gp(R.b64);
+#pragma unroll
+        for (int i = 0; i < NR; i++) {
+            int lr = NR * w + i, gy = row0 + i;
+            const bool in_tile = col_in && lr >= kJacH && lr < kJacH + kJacTY && gy >= 1 && gy < ncy;
+            b64t[i] = in_tile ? b64[gy * ncx + (bx * kJacTX + lane - kJacH)] : 0.0;
+        }
+    };
+    // FP64 residual of the tile's interior nodes at the current iterate,
+    // given the neighbouring waves' boundary rows, into slot rslot
+    auto tile_resid = [&](float below, float above) {
+        double rmax = 0.0;
+#pragma unroll
+        for (int i = 0; i < NR; i++) {
+            float W = lane_prev(v[i]);
+            float E = lane_next(v[i]);
+            float S = i > 0 ? v[i - 1] : below;
+            float N = i < NR - 1 ? v[i + 1] : above;
+            int lr = NR * w + i, gy = row0 + i;
+            if (col_in && lr >= kJacH && lr < kJacH + kJacTY && gy >= 1 && gy < ncy) {
+                double sum = (((b64t[i] + (double)W) + (double)E) + (double)N) + (double)S;
+                rmax = fmax(rmax, fabs(0.25 * (sum - 4.0 * (double)v[i])));
+            }
+        }
+        unsigned long long *slot = &R.resid[rslot * kRedSlots + (local & (kRedSlots - 1))];
+        block_max_atomic<kJacWaves>(rmax, shr, [&](double m) {
+            if (m > 0.0) atomicMax(slot, (unsigned long long)__double_as_longlong(m));
+        });
+    };
     // Only the Dirichlet rows 0 and ncy, and the rows outside the grid beyond
     // them, must keep their values.  The region's edge lanes and edge rows
     // lack a neighbour, so they are wrong after one sweep whether they keep
     // their value or take the (finite) update, and the valid region shrinks
     // alike.  Waves without such rows -- all but those of the first and last
     // tile row -- therefore sweep without a mask.
-    auto sweeps = [&](auto keep) {
+    auto sweeps = [&](auto keep, int ja, int jb) {
 #pragma unroll 1
-        for (int j = 0; j < nsw; j++) {
+        for (int j = ja; j < jb; j++) {
             float(*x)[2][kWave] = xch[j & 1];
             x[w + 1][0][lane] = v[0];
             x[w + 1][1][lane] = v[NR - 1];
@@ -281,46 +333,40 @@ __global__ __launch_bounds__(kJacThreads) void k_jacobi_block(const RunDev *rds,
             }
         }
     };
+    // the FP64 residual at local sweep j, through the exchange buffer the
+    // next sweep writes with the same rows
+    auto resid_at = [&](int j) {
+        float(*x)[2][kWave] = xch[j & 1];
+        x[w + 1][0][lane] = v[0];
+        x[w + 1][1][lane] = v[NR - 1];
+        __syncthreads();
+        tile_resid(x[w][1][lane], x[w + 2][0][lane]);
+    };
+    const int jres = rslot >= 0 ? rat : -1;  // local sweep of the residual, if any
+    auto run = [&](auto keep) {
+        if (jres > 0) {
+            sweeps(keep, 0, jres - 1);
+            load_b64();
+            sweeps(keep, jres - 1, jres);
+            if (jres < nsw) {
+                // the iterate at a check inside the launch: its residual, and
+                // the tile kept in the third buffer in case the solve ends there
+                resid_at(jres);
+                store_tile(gp(R.phi32[phi_third(phi_cur)]));
+                sweeps(keep, jres, nsw);
+            }
+        } else {
+            sweeps(keep, 0, nsw);
+        }
+    };
     const bool dirichlet_wave =
         __builtin_amdgcn_readfirstlane((row0 <= 0 || row0 + NR - 1 >= ncy) ? 1 : 0) != 0;
     if (dirichlet_wave)
-        sweeps([&](int i) { return ((upd >> i) & 1u) == 0u; });
+        run([&](int i) { return ((upd >> i) & 1u) == 0u; });
     else
-        sweeps([](int) { return false; });
-    const bool col_in = lane >= kJacH && lane < kJacH + kJacTX && bx * kJacTX + lane - kJacH < ncx;
-#pragma unroll
-    for (int i = 0; i < NR; i++) {
-        int lr = NR * w + i, gy = row0 + i;
-        if (col_in && lr >= kJacH && lr < kJacH + kJacTY && gy >= 1 && gy < ncy)
-            out[gy * ncx + (bx * kJacTX + lane - kJacH)] = v[i];
-    }
-    if (resid_slot < 0) return;
-    // FP64 residual of the tile's interior nodes after the last sweep
-    float(*x)[2][kWave] = xch[nsw & 1];
-    x[w + 1][0][lane] = v[0];
-    x[w + 1][1][lane] = v[NR - 1];
-    __syncthreads();
-    const float below = x[w][1][lane];
-    const float above = x[w + 2][0][lane];
-    const CUPIC_GLOBAL double *b64 = gp(R.b64);
-    double rmax = 0.0;
-#pragma unroll
-    for (int i = 0; i < NR; i++) {
-        float W = lane_prev(v[i]);
-        float E = lane_next(v[i]);
-        float S = i > 0 ? v[i - 1] : below;
-        float N = i < NR - 1 ? v[i + 1] : above;
-        int lr = NR * w + i, gy = row0 + i;
-        if (col_in && lr >= kJacH && lr < kJacH + kJacTY && gy >= 1 && gy < ncy) {
-            double sum = (((b64[gy * ncx + (bx * kJacTX + lane - kJacH)] + (double)W) + (double)E) +
-                          (double)N) + (double)S;
-            rmax = fmax(rmax, fabs(0.25 * (sum - 4.0 * (double)v[i])));
-        }
-    }
-    unsigned long long *slot = &R.resid[resid_slot * kRedSlots + (local & (kRedSlots - 1))];
-    block_max_atomic<kJacWaves>(rmax, shr, [&](double m) {
-        if (m > 0.0) atomicMax(slot, (unsigned long long)__double_as_longlong(m));
-    });
+        run([](int) { return false; });
+    store_tile(out);
+    if (jres == nsw) resid_at(nsw);  // the residual after the last sweep
 }
 
 // mode 2: phi64 += d, the correction solved by the last cycle of sweeps
@@ -361,7 +407,7 @@ __global__ __launch_bounds__(kBlock) void k_residual(const RunDev *rds, const in
                 const CUPIC_GLOBAL double *p = gp(R.phi64);
                 pc_ = p[k]; pw = p[kw]; pe = p[ke]; pn = p[k + ncx]; ps = p[k - ncx];
             } else {
-                const CUPIC_GLOBAL float *p = gp(R.phi32[(st->phi_cur + par) & 1]);
+                const CUPIC_GLOBAL float *p = gp(R.phi32[phi_at(st->phi_cur, par)]);
                 pc_ = p[k]; pw = p[kw]; pe = p[ke]; pn = p[k + ncx]; ps = p[k - ncx];
             }
             double sum = (((gp(R.b64)[k] + pw) + pe) + pn) + ps;
@@ -388,7 +434,7 @@ __global__ __launch_bounds__(kBlock) void k_solver_end(const RunDev *rds, const 
             st->last_resid = s > 0 ? (double)read_sweep_max(R, s - 1) : 0.0;
         else
             st->last_resid = read_resid(R, last_check);
-        st->phi_final = (st->phi_cur + par) & 1;
+        st->phi_final = phi_at(st->phi_cur, par);
         st->solve_done = 1;
     }
     if (st->solve_mode != 2) st->phi_cur = st->phi_final;
@@ -445,18 +491,18 @@ void launch_solver_begin(const Launch &L, int use_args, int mode, double tol, in
     LAUNCH_K(k_solver_begin, L, use_args, mode, tol, min_sweeps, max_sweeps, check_every);
 }
 void launch_jacobi_sweep(const Launch &L, int s) { LAUNCH_K(k_jacobi_sweep, L, s); }
-void launch_jacobi_block(const Launch &L, int halo, int s0, int nsw, int mode, int check_slot,
-                         int first, int par, int resid_slot)
+void launch_jacobi_block(const Launch &L, int halo, int s0, int nsw, int mode, int first,
+                         int par, int cslot, int cat, int cmid, int rslot, int rat)
 {
     if (L.nblocks <= 0) return;
     if (halo == kJacHNarrow)
         hipLaunchKernelGGL(k_jacobi_block<kJacHNarrow>, dim3(L.nblocks), dim3(kJacThreads), 0,
-                           L.stream, L.rds, L.start, L.nruns, s0, nsw, mode, check_slot, first,
-                           par, resid_slot);
+                           L.stream, L.rds, L.start, L.nruns, s0, nsw, mode, first, par, cslot,
+                           cat, cmid, rslot, rat);
     else
         hipLaunchKernelGGL(k_jacobi_block<kJacH>, dim3(L.nblocks), dim3(kJacThreads), 0,
-                           L.stream, L.rds, L.start, L.nruns, s0, nsw, mode, check_slot, first,
-                           par, resid_slot);
+                           L.stream, L.rds, L.start, L.nruns, s0, nsw, mode, first, par, cslot,
+                           cat, cmid, rslot, rat);
 }
 void launch_ir_update(const Launch &L, int par) { LAUNCH_K(k_ir_update, L, par); }
 void launch_residual(const Launch &L, int c, int mode, int par)

@@ -179,6 +179,66 @@ def test_jacobi_matches_fp64_sweeps(dev, mode, grid):
     assert H.rel_err(Ey, fy.reshape(Ey.shape)) < 1e-6
 
 
+@pytest.mark.parametrize("grid", [(64, 32, 10), (4096, 257, 1)], ids=["64x32", "4096x257"])
+def test_jacobi_stops_at_the_oracle_check(dev, grid):
+    """Mode 1 stopping inside a blocked launch: with the tolerance between the
+    FP64 residuals at sweeps 150 and 200, the GPU solve stops at sweep 200 as
+    the FP64 oracle does (the check at 200 falls inside a launch of 15, or of
+    7, sweeps) and returns the potential of that sweep."""
+    import ctypes as C
+    ncx, ncy, ppc = grid
+    p = small(ncx=ncx, ncy=ncy, ppc=ppc, jacobi_mode=1, jacobi_max_sweeps=300,
+              jacobi_check_every=50)
+    run = cp.Run(p)
+    run.init_sim()
+    run.step(p.dt)
+    acc = H.periodic(run.mesh(cp.MESH_CHARGE)).astype(np.float64)
+    rho = acc * 2.0**-32 / p.ds**3
+    rho[0] *= 2.0
+    rho[-1] *= 2.0
+    phi0 = np.zeros((p.ncy + 1, p.ncx))
+    O.lib().orc_phi_linear(p.ncx, p.ncy, p.phi_p, phi0.ravel())
+    jj, ii = np.mgrid[0:p.ncy + 1, 0:p.ncx]
+    phi0[1:-1] += 0.3 * np.sin(2 * np.pi * ii[1:-1] / p.ncx) * np.sin(np.pi * jj[1:-1] / p.ncy)
+
+    def oracle(tol, max_sweeps):
+        want = phi0.ravel().copy()
+        r = C.c_double()
+        n = O.lib().orc_poisson_solve(p.ncx, p.ncy, p.ds, p.eps0, rho.ravel().copy(), want, 1,
+                                      tol, 0, max_sweeps, 50, C.byref(r))
+        return n, want, r.value
+
+    _, _, r150 = oracle(0.0, 150)
+    _, _, r200 = oracle(0.0, 200)
+    tol = (r150 * r200) ** 0.5
+    n, want, r = oracle(tol, 300)
+    assert n == 200 and r200 < tol < r150
+    run.set_phi(np.concatenate([phi0, phi0[:, :1]], 1))
+    run.charge_deposition()
+    sw, res = run.poisson_solver(tol=tol)
+    say("tol", tol, "sweeps", sw, "resid gpu", res, "oracle", r)
+    assert sw == 200
+    assert res == pytest.approx(r, rel=0.2, abs=2e-7)
+    got = H.periodic(run.mesh(cp.MESH_PHI))
+    assert H.rel_err(got, want.reshape(got.shape)) < 2e-6
+    # the next solve starts from that potential (the third buffer) and the
+    # field is taken from it
+    run.field_solver()
+    fx = np.zeros(got.size)
+    fy = np.zeros(got.size)
+    O.lib().orc_field(p.ncx, p.ncy, p.ds, got.ravel().copy(), fx, fy)
+    assert H.rel_err(H.periodic(run.mesh(cp.MESH_EY)), fy.reshape(got.shape)) < 1e-6
+    sw2, _ = run.poisson_solver(tol=0.0)
+    assert sw2 == 300
+    want2 = want.copy()
+    r2 = C.c_double()
+    assert O.lib().orc_poisson_solve(p.ncx, p.ncy, p.ds, p.eps0, rho.ravel().copy(), want2, 1,
+                                     0.0, 0, 300, 50, C.byref(r2)) == 300
+    got2 = H.periodic(run.mesh(cp.MESH_PHI))
+    assert H.rel_err(got2, want2.reshape(got2.shape)) < 2e-6
+    run.close()
+
+
 def test_refinement_reaches_fp64_tolerance(dev):
     """Mode 2 drives the FP64 residual below 1e-6 where it is reachable
     (small grid, generous cap) and reports the sweeps the oracle needs."""
