@@ -7,7 +7,7 @@
 //               push-hint byte per particle slot.
 //   grids       periodic in x, [ncy+1][ncx]: node (i, j) at j*ncx + i.
 //               charge: int64 fixed point (2^-32 e); rhs: FP32 and FP64;
-//               phi: FP32 ping-pong (modes 0, 1) / FP64 master + FP32
+//               phi: FP32, three buffers (modes 0, 1) / FP64 master + FP32
 //               correction ping-pong (mode 2) / FP64 (mode 3, direct);
 //               E: float2 (Ex, Ey).
 //   direct      mode 3 only: DFT and sine bases, eigenvalue tables, transformed
