@@ -243,31 +243,40 @@ void seq_solve(const Target &T, int use_args, int mode, double tol, int min_swee
     int c = 0, s = 0, nlaunch = 0, dpar = 0;
     if (mode == 1) {
         // FP64 residual checks at sweeps 0, check, 2 check, ... and
-        // max_sweeps, check k in slot k: the first here, the others reduced
-        // by the launch whose sweeps reach them.  Launches of `per` sweeps
-        // run straight through the check points; a check inside a launch
-        // keeps its iterate in the third phi buffer, and the next launch
-        // reads the check and ends the solve there if it is met.
-        launch_residual(N, 0, mode, 0);
+        // max_sweeps, check k in slot k, each reduced by the launch whose
+        // sweeps reach it (the first one too, from its input, unless a
+        // launch spans a whole check interval; then a separate kernel).
+        // Launches of `per` sweeps run straight through the check points: a
+        // check inside a launch keeps its iterate in the third phi buffer,
+        // and the next launch reads the check and ends the solve there if it
+        // is met.
         const int step = std::min(per, check);  // so at most one check per launch
-        int cslot = 0, cat = 0, cmid = 0;       // the check the next launch reads
+        const bool fuse0 = check > per && max_sweeps > per;
+        if (!fuse0) launch_residual(N, 0, mode, 0);
+        int cslot = fuse0 ? -1 : 0, cat = 0, cpar = 0;  // the check the next launch reads
         while (s < max_sweeps) {
             const int nsw = std::min(step, max_sweeps - s);
-            const int next_at = std::min((c + 1) * check, max_sweeps);
-            int rslot = -1, rat = 0;
-            if (next_at <= s + nsw) {
-                rslot = c + 1;
-                rat = next_at - s;
+            int rslot = -1, rat = 0, at = 0;
+            if (fuse0 && nlaunch == 0) {
+                rslot = 0;
+            } else {
+                at = std::min((c + 1) * check, max_sweeps);
+                if (at <= s + nsw) {
+                    rslot = c + 1;
+                    rat = at - s;
+                }
             }
-            launch_jacobi_block(B, halo, s, nsw, 1, 0, nlaunch & 1, cslot, cat, cmid, rslot, rat);
-            nlaunch++;
+            launch_jacobi_block(B, halo, s, nsw, 1, 0, nlaunch & 1, cslot, cat, cpar, rslot, rat);
             cslot = -1;
             if (rslot >= 0) {
-                c += 1;
-                cslot = c;
-                cat = next_at;
-                cmid = rat < nsw ? 1 : 0;
+                c = rslot;
+                cslot = rslot;
+                cat = at;
+                // where the checked iterate is: this launch's input, its
+                // output (the next launch's input) or the third buffer
+                cpar = rat == 0 ? nlaunch : rat == nsw ? nlaunch + 1 : -1;
             }
+            nlaunch++;
             s += nsw;
         }
         launch_solver_end(T.L(kShapeRun), c, nlaunch & 1);
@@ -282,7 +291,7 @@ void seq_solve(const Target &T, int use_args, int mode, double tol, int min_swee
         for (int q = 0; q < k;) {
             int nsw = std::min(per, k - q);
             launch_jacobi_block(B, halo, s + q, nsw, 2, q == 0 ? 1 : 0, dl & 1, q == 0 ? c : -1,
-                                s + q, 0, -1, 0);
+                                s + q, dl & 1, -1, 0);
             dl++;
             q += nsw;
         }

@@ -49,7 +49,7 @@ void launch_solver_begin(const Launch &L, int use_args, int mode, double tol, in
                          int max_sweeps, int check_every);
 void launch_jacobi_sweep(const Launch &L, int s);
 void launch_jacobi_block(const Launch &L, int halo, int s0, int nsw, int mode, int first,
-                         int par, int cslot, int cat, int cmid, int rslot, int rat);
+                         int par, int cslot, int cat, int cpar, int rslot, int rat);
 void launch_ir_update(const Launch &L, int par);
 void launch_residual(const Launch &L, int c, int mode, int par);
 void launch_solver_end(const Launch &L, int last_check, int par);

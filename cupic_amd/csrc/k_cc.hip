@@ -317,10 +317,20 @@ __global__ __launch_bounds__(kBlock) void k_far_rank(const RunDev *rds, const in
         int b = R.base[s][9 * R.ncells + key];
         int n = R.off[s][st->cur ^ 1][key + 1] - b;
         const int *slots = (const int *)R.part[s][st->cur ^ 1];
+        // four slots per round trip: their entry indices, then the entries'
+        // input indices (an index outside the list reads entry 0 and counts
+        // for nothing)
         int rank = 0;
-        for (int u = 0; u < n; u++) {
-            int q = slots[4 * (b + u)];
-            if ((unsigned)q < (unsigned)F) rank += far[q].idx < idx;
+        for (int u = 0; u < n; u += 4) {
+            int q[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) q[k] = u + k < n ? slots[4 * (b + u + k)] : -1;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const bool ok = (unsigned)q[k] < (unsigned)F;
+                const int other = far[ok ? q[k] : 0].idx;
+                rank += (ok && other < idx) ? 1 : 0;
+            }
         }
         R.far[s][e].rank = rank;
     }

@@ -53,10 +53,15 @@ __global__ __launch_bounds__(kBlock) void k_solver_begin(const RunDev *rds, cons
     RUN_PROLOGUE();
     int mx = use_args ? max_sweeps : R.step_max;
     if (mx > R.max_sweeps_alloc) mx = R.max_sweeps_alloc;
-    for (int q = threadIdx.x; q < (R.max_sweeps_alloc + 1) * kRedSlots; q += kBlock)
-        R.sweep_max[q] = 0u;
-    for (int q = threadIdx.x; q < (R.max_sweeps_alloc + 2) * kRedSlots; q += kBlock)
-        R.resid[q] = 0ull;
+    const int md = use_args ? mode : R.step_mode;
+    const int ck = max(1, use_args ? check_every : R.step_check);
+    // clear the reduction slots this solve uses: one per sweep (mode 0), one
+    // per residual check (modes 1, 2: sweeps 0, ck, 2 ck, ... and mx), two
+    // for mode 3
+    const int nsweep = md == 0 ? mx + 1 : 0;
+    const int nres = md == 0 ? 0 : min(R.max_sweeps_alloc + 2, (mx + ck - 1) / ck + 2);
+    for (int q = threadIdx.x; q < nsweep * kRedSlots; q += kBlock) R.sweep_max[q] = 0u;
+    for (int q = threadIdx.x; q < nres * kRedSlots; q += kBlock) R.resid[q] = 0ull;
     if (threadIdx.x == 0) {
         st->solve_mode = use_args ? mode : R.step_mode;
         st->solve_tol = use_args ? tol : R.step_tol;
@@ -91,15 +96,15 @@ __device__ __forceinline__ float read_sweep_max(const RunDev &R, int s)
 }
 
 // Ends a solve at sweep s with residual res; the solution is in phi buffer
-// phi_at(phi_cur, par), or in the third buffer when `third` (a check inside
+// phi_at(phi_cur, par), or in the third buffer when par < 0 (a check inside
 // a blocked launch).
 __device__ __forceinline__ void mark_done(const RunDev &R, RunState *st, int local, int s,
-                                          double res, int par, bool third = false)
+                                          double res, int par)
 {
     if (local == 0 && threadIdx.x == 0) {
         st->sweeps_done = s;
         st->last_resid = res;
-        st->phi_final = third ? phi_third(st->phi_cur) : phi_at(st->phi_cur, par);
+        st->phi_final = par < 0 ? phi_third(st->phi_cur) : phi_at(st->phi_cur, par);
         st->solve_done = 1;
     }
 }
@@ -194,15 +199,16 @@ __device__ __forceinline__ float lane_next(float v)
 // and selects the input buffer; the output goes to the other one.
 // Residual checks (mode 1): with cslot >= 0 the launch first reads the FP64
 // residual of the check at sweep cat and ends the solve if it is met; that
-// iterate is the launch's input, or the third phi buffer when the check fell
-// inside the previous launch (cmid).  With rslot >= 0 the launch reduces the
-// FP64 residual of its tile after rat of its sweeps into slot rslot; when
-// rat < nsw it also stores the tile at that sweep into the third buffer.
+// iterate is in buffer phi_at(phi_cur, cpar), or in the third buffer when
+// cpar < 0 (the check fell inside the previous launch).  With rslot >= 0 the
+// launch reduces the FP64 residual of its tile after rat of its sweeps (0:
+// its input) into slot rslot; when 0 < rat < nsw it also stores the tile at
+// that sweep into the third buffer.
 template <int H>
 __global__ __launch_bounds__(kJacThreads) void k_jacobi_block(const RunDev *rds, const int *start,
                                                          int nruns, int s0, int nsw, int mode,
                                                          int first, int par, int cslot, int cat,
-                                                         int cmid, int rslot, int rat)
+                                                         int cpar, int rslot, int rat)
 {
     RUN_PROLOGUE();
     constexpr int NR = kJacRows;
@@ -218,7 +224,7 @@ __global__ __launch_bounds__(kJacThreads) void k_jacobi_block(const RunDev *rds,
     const double res = cslot >= 0 ? read_resid(R, cslot) : 0.0;
     if (done) return;
     if (cslot >= 0 && (res <= tol || cat >= smax)) {
-        mark_done(R, st, local, cat, res, par, cmid != 0);
+        mark_done(R, st, local, cat, res, cpar);
         return;
     }
     if (s0 >= smax) return;
@@ -344,7 +350,12 @@ __global__ __launch_bounds__(kJacThreads) void k_jacobi_block(const RunDev *rds,
     };
     const int jres = rslot >= 0 ? rat : -1;  // local sweep of the residual, if any
     auto run = [&](auto keep) {
-        if (jres > 0) {
+        if (jres == 0) {
+            // the residual of the input (the solve's first check)
+            load_b64();
+            resid_at(0);
+            sweeps(keep, 0, nsw);
+        } else if (jres > 0) {
             sweeps(keep, 0, jres - 1);
             load_b64();
             sweeps(keep, jres - 1, jres);
@@ -492,17 +503,17 @@ void launch_solver_begin(const Launch &L, int use_args, int mode, double tol, in
 }
 void launch_jacobi_sweep(const Launch &L, int s) { LAUNCH_K(k_jacobi_sweep, L, s); }
 void launch_jacobi_block(const Launch &L, int halo, int s0, int nsw, int mode, int first,
-                         int par, int cslot, int cat, int cmid, int rslot, int rat)
+                         int par, int cslot, int cat, int cpar, int rslot, int rat)
 {
     if (L.nblocks <= 0) return;
     if (halo == kJacHNarrow)
         hipLaunchKernelGGL(k_jacobi_block<kJacHNarrow>, dim3(L.nblocks), dim3(kJacThreads), 0,
                            L.stream, L.rds, L.start, L.nruns, s0, nsw, mode, first, par, cslot,
-                           cat, cmid, rslot, rat);
+                           cat, cpar, rslot, rat);
     else
         hipLaunchKernelGGL(k_jacobi_block<kJacH>, dim3(L.nblocks), dim3(kJacThreads), 0,
                            L.stream, L.rds, L.start, L.nruns, s0, nsw, mode, first, par, cslot,
-                           cat, cmid, rslot, rat);
+                           cat, cpar, rslot, rat);
 }
 void launch_ir_update(const Launch &L, int par) { LAUNCH_K(k_ir_update, L, par); }
 void launch_residual(const Launch &L, int c, int mode, int par)
