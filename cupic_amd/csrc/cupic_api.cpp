@@ -244,23 +244,28 @@ void seq_solve(const Target &T, int use_args, int mode, double tol, int min_swee
     if (mode == 1) {
         // FP64 residual checks at sweeps 0, check, 2 check, ... and
         // max_sweeps, check k in slot k, each reduced by the launch whose
-        // sweeps reach it (the first one too, from its input, unless a
-        // launch spans a whole check interval; then a separate kernel).
-        // Launches of `per` sweeps run straight through the check points: a
-        // check inside a launch keeps its iterate in the third phi buffer,
-        // and the next launch reads the check and ends the solve there if it
-        // is met.
+        // sweeps reach it.  With the wide halo (one run, short launches
+        // whose fixed cost dominates) launches of `per` sweeps run straight
+        // through the check points: a check inside a launch keeps its iterate
+        // in the third phi buffer, the next launch reads the check and ends
+        // the solve there if it is met, and the first check is reduced by the
+        // first launch from its input.  With the narrow halo (a batch whose
+        // launches fill the device many times over) launches end at every
+        // check, which keeps the instance with the inside check, and its
+        // registers, out of the way.
+        const bool through = !narrow;
         const int step = std::min(per, check);  // so at most one check per launch
-        const bool fuse0 = check > per && max_sweeps > per;
+        const bool fuse0 = through && check > per && max_sweeps > per;
         if (!fuse0) launch_residual(N, 0, mode, 0);
         int cslot = fuse0 ? -1 : 0, cat = 0, cpar = 0;  // the check the next launch reads
         while (s < max_sweeps) {
-            const int nsw = std::min(step, max_sweeps - s);
+            int nsw = std::min(step, max_sweeps - s);
             int rslot = -1, rat = 0, at = 0;
             if (fuse0 && nlaunch == 0) {
                 rslot = 0;
             } else {
                 at = std::min((c + 1) * check, max_sweeps);
+                if (!through) nsw = std::min(nsw, at - s);
                 if (at <= s + nsw) {
                     rslot = c + 1;
                     rat = at - s;

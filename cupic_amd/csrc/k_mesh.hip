@@ -204,7 +204,12 @@ __device__ __forceinline__ float lane_next(float v)
 // launch reduces the FP64 residual of its tile after rat of its sweeps (0:
 // its input) into slot rslot; when 0 < rat < nsw it also stores the tile at
 // that sweep into the third buffer.
-template <int H>
+// RK: 0, the launch reduces no residual; 1, at its start (rat = 0) or end
+// (rat = nsw); 2, inside it (0 < rat < nsw, with the third-buffer copy and
+// the FP64 rhs fetched a sweep ahead).  Separate instances keep the register
+// count of the common ones low (53 VGPRs for RK 0, against 80 for RK 2),
+// which matters to batches, whose launches fill the device many times.
+template <int H, int RK>
 __global__ __launch_bounds__(kJacThreads) void k_jacobi_block(const RunDev *rds, const int *start,
                                                          int nruns, int s0, int nsw, int mode,
                                                          int first, int par, int cslot, int cat,
@@ -348,25 +353,24 @@ __global__ __launch_bounds__(kJacThreads) void k_jacobi_block(const RunDev *rds,
         __syncthreads();
         tile_resid(x[w][1][lane], x[w + 2][0][lane]);
     };
-    const int jres = rslot >= 0 ? rat : -1;  // local sweep of the residual, if any
+    const int jres = RK > 0 && rslot >= 0 ? rat : -1;  // local sweep of the residual, if any
     auto run = [&](auto keep) {
-        if (jres == 0) {
-            // the residual of the input (the solve's first check)
-            load_b64();
-            resid_at(0);
-            sweeps(keep, 0, nsw);
-        } else if (jres > 0) {
+        if (RK == 2) {
+            // a check inside the launch: its residual, with the rhs fetched
+            // during the sweep before, and the tile kept in the third buffer
+            // in case the solve ends there
             sweeps(keep, 0, jres - 1);
             load_b64();
             sweeps(keep, jres - 1, jres);
-            if (jres < nsw) {
-                // the iterate at a check inside the launch: its residual, and
-                // the tile kept in the third buffer in case the solve ends there
-                resid_at(jres);
-                store_tile(gp(R.phi32[phi_third(phi_cur)]));
-                sweeps(keep, jres, nsw);
-            }
+            resid_at(jres);
+            store_tile(gp(R.phi32[phi_third(phi_cur)]));
+            sweeps(keep, jres, nsw);
         } else {
+            if (RK == 1 && jres == 0) {
+                // the residual of the input (the solve's first check)
+                load_b64();
+                resid_at(0);
+            }
             sweeps(keep, 0, nsw);
         }
     };
@@ -377,7 +381,10 @@ __global__ __launch_bounds__(kJacThreads) void k_jacobi_block(const RunDev *rds,
     else
         run([](int) { return false; });
     store_tile(out);
-    if (jres == nsw) resid_at(nsw);  // the residual after the last sweep
+    if (RK == 1 && jres == nsw) {  // the residual after the last sweep
+        load_b64();
+        resid_at(nsw);
+    }
 }
 
 // mode 2: phi64 += d, the correction solved by the last cycle of sweeps
@@ -506,14 +513,21 @@ void launch_jacobi_block(const Launch &L, int halo, int s0, int nsw, int mode, i
                          int par, int cslot, int cat, int cpar, int rslot, int rat)
 {
     if (L.nblocks <= 0) return;
-    if (halo == kJacHNarrow)
-        hipLaunchKernelGGL(k_jacobi_block<kJacHNarrow>, dim3(L.nblocks), dim3(kJacThreads), 0,
-                           L.stream, L.rds, L.start, L.nruns, s0, nsw, mode, first, par, cslot,
-                           cat, cpar, rslot, rat);
-    else
-        hipLaunchKernelGGL(k_jacobi_block<kJacH>, dim3(L.nblocks), dim3(kJacThreads), 0,
-                           L.stream, L.rds, L.start, L.nruns, s0, nsw, mode, first, par, cslot,
-                           cat, cpar, rslot, rat);
+#define JAC_LAUNCH(HH, RR)                                                                    \
+    hipLaunchKernelGGL((k_jacobi_block<HH, RR>), dim3(L.nblocks), dim3(kJacThreads), 0, L.stream, \
+                       L.rds, L.start, L.nruns, s0, nsw, mode, first, par, cslot, cat, cpar,    \
+                       rslot, rat)
+    const int rk = rslot < 0 ? 0 : (rat > 0 && rat < nsw) ? 2 : 1;
+    if (halo == kJacHNarrow) {
+        if (rk == 2) JAC_LAUNCH(kJacHNarrow, 2);
+        else if (rk == 1) JAC_LAUNCH(kJacHNarrow, 1);
+        else JAC_LAUNCH(kJacHNarrow, 0);
+    } else {
+        if (rk == 2) JAC_LAUNCH(kJacH, 2);
+        else if (rk == 1) JAC_LAUNCH(kJacH, 1);
+        else JAC_LAUNCH(kJacH, 0);
+    }
+#undef JAC_LAUNCH
 }
 void launch_ir_update(const Launch &L, int par) { LAUNCH_K(k_ir_update, L, par); }
 void launch_residual(const Launch &L, int c, int mode, int par)
