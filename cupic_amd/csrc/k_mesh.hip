@@ -195,8 +195,9 @@ __device__ __forceinline__ float lane_next(float v)
 // sweep and still covers the tile and one ring after nsw sweeps, so the
 // iterates equal those of nsw single sweeps (same operations per node).
 // Mode 1 relaxes phi (rhs b), mode 2 the refinement correction d (rhs r),
-// d starting at 0 when first.  par counts the launches of the solve so far
-// and selects the input buffer; the output goes to the other one.
+// d starting at 0 when first.  par counts the launches so far (of the solve
+// in mode 1, of the correction cycle in mode 2) and selects the input
+// buffer; the output goes to the other one.
 // Residual checks (mode 1): with cslot >= 0 the launch first reads the FP64
 // residual of the check at sweep cat and ends the solve if it is met; that
 // iterate is in buffer phi_at(phi_cur, cpar), or in the third buffer when
