@@ -228,6 +228,11 @@ def test_jacobi_stops_at_the_oracle_check(dev, grid):
     fy = np.zeros(got.size)
     O.lib().orc_field(p.ncx, p.ncy, p.ds, got.ravel().copy(), fx, fy)
     assert H.rel_err(H.periodic(run.mesh(cp.MESH_EY)), fy.reshape(got.shape)) < 1e-6
+    # solved again at the same tolerance: the first check (sweep 0, reduced
+    # by the first launch from its input) stops it with the potential as is
+    sw1, res1 = run.poisson_solver(tol=tol)
+    assert sw1 == 0 and res1 == res
+    assert np.array_equal(H.periodic(run.mesh(cp.MESH_PHI)), got)
     sw2, _ = run.poisson_solver(tol=0.0)
     assert sw2 == 300
     want2 = want.copy()
