@@ -485,6 +485,26 @@ def main():
                                     "formula": "34*(Ne+Ni) + nodes*(32 + 12*S): push_rank reads "
                                                "the 1 B push hint, scatter reads 16 B and writes "
                                                "16 B + the 1 B hint"}
+    if kern:
+        # the same bytes split between the particle kernels and the grid
+        # kernels, each over its own kernels' device time per step (the
+        # per-class times of the breakdown steps)
+        per = out["kernels_ms_per_launch"]
+        part_ms = sum(per[k] for k in ("push_rank", "inject", "scan", "scatter", "finish"))
+        grid_ms = sum(per[k] for k in ("deposit", "solve", "field"))
+        grid_bytes = nodes * (32.0 + 12.0 * S)
+        out["particle_kernels_roofline"] = {
+            "ms_per_step": part_ms,
+            "frac": 84.0 * npart / (part_ms * 1e-3) / HBM_PEAK,
+            "frac_design": 34.0 * npart / (part_ms * 1e-3) / HBM_PEAK,
+            "formula": "84*(Ne+Ni) (design: 34*(Ne+Ni)) over push_rank + inject + scan + "
+                       "scatter + finish"}
+        out["grid_kernels_roofline"] = {
+            "ms_per_step": grid_ms,
+            "frac": grid_bytes / (grid_ms * 1e-3) / HBM_PEAK,
+            "formula": "nodes*(32 + 12*S) over deposit + solve + field, against the HBM "
+                       "peak; the Jacobi grids stay in the 256 MB Infinity Cache, which "
+                       "serves most of these bytes"}
     out["probe"] = {"total_absorbed": [int(v) for v in tally["total_probe"]],
                     "charge_collected": tally["charge"]}
     run.close()
