@@ -48,7 +48,7 @@ constexpr int kJacThreads = kJacWaves * 64;
 // single run, whose launches are short; a batch of many runs, whose
 // launches fill the device many times over, takes the narrow one.
 constexpr int kJacH = 16;
-constexpr int kJacHNarrow = 8;
+constexpr int kJacHNarrow = 10;
 constexpr int kJacNarrowMinBlocks = 1024;   // wide-halo workgroups from which to go narrow
 __host__ __device__ constexpr int jac_tx(int h) { return 64 - 2 * h; }
 __host__ __device__ constexpr int jac_ty(int h) { return kJacRows * kJacWaves - 2 * h; }

@@ -181,10 +181,12 @@ def test_jacobi_matches_fp64_sweeps(dev, mode, grid):
 
 @pytest.mark.parametrize("grid", [(64, 32, 10), (4096, 257, 1)], ids=["64x32", "4096x257"])
 def test_jacobi_stops_at_the_oracle_check(dev, grid):
-    """Mode 1 stopping inside a blocked launch: with the tolerance between the
+    """Mode 1 stopping at a residual check: with the tolerance between the
     FP64 residuals at sweeps 150 and 200, the GPU solve stops at sweep 200 as
-    the FP64 oracle does (the check at 200 falls inside a launch of 15, or of
-    7, sweeps) and returns the potential of that sweep."""
+    the FP64 oracle does and returns the potential of that sweep.  With the
+    wide halo (64 x 32) the check falls inside a launch of 15 sweeps and the
+    potential comes from the third buffer; with the narrow halo (4096 x 257)
+    launches end at every check."""
     import ctypes as C
     ncx, ncy, ppc = grid
     p = small(ncx=ncx, ncy=ncy, ppc=ppc, jacobi_mode=1, jacobi_max_sweeps=300,
