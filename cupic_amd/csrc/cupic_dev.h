@@ -406,6 +406,27 @@ struct Pushed {
     int dir;           // 0..8 near, kDirFar, kDirProbe, kDirTop
 };
 
+// New cell, in-cell fractions and direction of a particle pushed to
+// (o.x, o.y) from cell (oic, ojc), without branches: the direction of an
+// absorbed particle (y < 0: probe, y >= Ly: top) overrides the cell's, and
+// its cell fields, clamped by cell_of, are not read.
+template <class G>
+__device__ __forceinline__ void classify(const G &R, Pushed &o, int oic, int ojc)
+{
+    float fx, fy;
+    int nic, njc;
+    cell_of(R, o.x, o.y, fx, fy, nic, njc);
+    o.fx = fx; o.fy = fy; o.nic = nic; o.njc = njc;
+    int di = nic - oic;
+    const int dj = njc - ojc;
+    di = di == R.ncx - 1 ? -1 : (di == -(R.ncx - 1) ? 1 : di);
+    o.di = di;
+    const float ty = o.y * R.inv_ds;
+    const bool near = di >= -1 && di <= 1 && dj >= -1 && dj <= 1;
+    const int dcell = near ? (di + 1) + 3 * (dj + 1) : (int)kDirFar;
+    o.dir = ty < 0.0f ? (int)kDirProbe : (ty >= (float)R.ncy ? (int)kDirTop : dcell);
+}
+
 // Leapfrog push of one particle (src/particles.cu:110-231 semantics),
 // periodic wrap into [0, Lx) (src/cc.cu:775-810, Q6), absorption at y < 0
 // (probe) and y >= Ly (top) (src/cc.cu:182-185), new cell and direction.
@@ -425,21 +446,7 @@ __device__ __forceinline__ Pushed push_one(const G &R, const float2 *__restrict_
     if (nx >= R.Lx_f) nx = nx - R.Lx_f;
     o.x = nx; o.y = ny; o.vx = nvx; o.vy = nvy;
     o.oic = oic; o.ojc = ojc;
-    o.fx = 0.0f; o.fy = 0.0f; o.nic = 0; o.njc = 0; o.di = 0;
-    float ty = ny * R.inv_ds;
-    if (ty < 0.0f) { o.dir = kDirProbe; return o; }
-    if (ty >= (float)R.ncy) { o.dir = kDirTop; return o; }
-    int nic, njc;
-    cell_of(R, nx, ny, fx, fy, nic, njc);
-    o.fx = fx; o.fy = fy; o.nic = nic; o.njc = njc;
-    int di = nic - oic, dj = njc - ojc;
-    if (di == R.ncx - 1) di = -1;
-    else if (di == -(R.ncx - 1)) di = 1;
-    o.di = di;
-    if (di >= -1 && di <= 1 && dj >= -1 && dj <= 1)
-        o.dir = (di + 1) + 3 * (dj + 1);
-    else
-        o.dir = kDirFar;
+    classify(R, o, oic, ojc);
     return o;
 }
 

@@ -407,21 +407,7 @@ __device__ __forceinline__ Pushed push_tile(const Geo &R, const float2 *Et, int 
     if (nx >= R.Lx_f) nx = nx - R.Lx_f;
     o.x = nx; o.y = ny; o.vx = nvx; o.vy = nvy;
     o.oic = oic; o.ojc = ojc;
-    o.fx = 0.0f; o.fy = 0.0f; o.nic = 0; o.njc = 0; o.di = 0;
-    float ty = ny * R.inv_ds;
-    if (ty < 0.0f) { o.dir = kDirProbe; return o; }
-    if (ty >= (float)R.ncy) { o.dir = kDirTop; return o; }
-    int nic, njc;
-    cell_of(R, nx, ny, fx, fy, nic, njc);
-    o.fx = fx; o.fy = fy; o.nic = nic; o.njc = njc;
-    int di = nic - oic, dj = njc - ojc;
-    if (di == R.ncx - 1) di = -1;
-    else if (di == -(R.ncx - 1)) di = 1;
-    o.di = di;
-    if (di >= -1 && di <= 1 && dj >= -1 && dj <= 1)
-        o.dir = (di + 1) + 3 * (dj + 1);
-    else
-        o.dir = kDirFar;
+    classify(R, o, oic, ojc);
     return o;
 }
 
