@@ -679,15 +679,26 @@ __device__ __forceinline__ void scatter_task(const RunDev &R, RunState *st, cons
     bool bad = false;
     unsigned long long acc1 = 0ull, acc2 = 0ull;  // stayer sums of old cell acc_cl
     int acc_cl = -1, acc_n = 0;
-    // the next chunk is in flight while one is processed
+    // The next chunk is in flight while one is processed.  A chunk's stores
+    // are issued during the next chunk, after its prefetch: loads and stores
+    // share one wait counter here, so the wait for a prefetch also waits for
+    // every store issued before it, and stores one chunk old have completed.
     v4f nxt = v4f{0.0f, 0.0f, 0.0f, 0.0f};
     if (p0 + lane < p1) nxt = ld4(PIN, p0 + lane);
+    v4f prec = v4f{0.0f, 0.0f, 0.0f, 0.0f};  // the previous chunk's store
+    int pdest = -1;
+    unsigned phint = 0u;
     for (int i0 = p0; i0 < p1; i0 += kWave) {
         const int i = i0 + lane;
         const bool valid = i < p1;
         const v4f cur4 = nxt;
         const float x = cur4.x, y = cur4.y, vx = cur4.z, vy = cur4.w;
         if (i + kWave < p1) nxt = ld4(PIN, i + kWave);
+        if (pdest >= 0) {
+            st4(POUT, pdest, prec);
+            HOUT[pdest] = (unsigned char)phint;
+        }
+        pdest = -1;
         bool in_strip;
         Pushed o = push_tile(G, Et, nc, col0, j0, E, qdtm, x, y, vx, vy, in_strip);
         int dir = valid ? o.dir : (int)kDirNone;
@@ -742,8 +753,9 @@ __device__ __forceinline__ void scatter_task(const RunDev &R, RunState *st, cons
         if (dir < 9) {
             int dest = rank;  // counters hold slot base + rank
             if ((unsigned)dest < (unsigned)cap) {
-                st4(POUT, dest, v4f{o.x, o.y, o.vx, o.vy});
-                HOUT[dest] = (unsigned char)push_hint(G, o.fx, o.fy, o.vx, o.vy, o.nic, o.njc);
+                prec = v4f{o.x, o.y, o.vx, o.vy};
+                pdest = dest;
+                phint = push_hint(G, o.fx, o.fy, o.vx, o.vy, o.nic, o.njc);
             } else {
                 bad = true;
             }
@@ -796,6 +808,10 @@ __device__ __forceinline__ void scatter_task(const RunDev &R, RunState *st, cons
             int key = (o.njc - (j0 - 1)) * TW + (cl + o.di + 1);
             tile_add(tile, TW, key, v1, v2, sgn);
         }
+    }
+    if (pdest >= 0) {
+        st4(POUT, pdest, prec);
+        HOUT[pdest] = (unsigned char)phint;
     }
     if (acc_cl >= 0) flush_stayers(tile, TW, acc_cl, acc1, acc2, sgn, lane);
     wave_lds_sync();
