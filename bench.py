@@ -135,11 +135,12 @@ def draw_batched(n_runs, seed):
     return out
 
 
-def batched_leg(cp, n_runs, steps, seq_steps, seed):
+def batched_leg(cp, n_runs, steps, seq_steps, seed, fixed_tile=0):
     """One HIP-graph step advances all runs; baseline: the same runs stepped
-    one after another through the plain path (cupic_step per run).  The tile
-    (cells per wave strip) is tuned in place on the batch itself: each
-    candidate is set on every run and timed over 2 batched steps."""
+    one after another through the plain path (cupic_step per run), skipped
+    when seq_steps is 0.  The tile (cells per wave strip) is tuned in place
+    on the batch itself: each candidate is set on every run and timed over 2
+    batched steps, unless fixed_tile names one."""
     draws = draw_batched(n_runs, seed)
     runs = []
     for d in draws:
@@ -154,7 +155,7 @@ def batched_leg(cp, n_runs, steps, seq_steps, seed):
     print(f"# batched: {n_runs} runs, {npart} particles", file=sys.stderr, flush=True)
     tune = {}
     done = 0
-    for tile in (0, 2, 4, 6, 8, 12, 16):
+    for tile in ((fixed_tile,) if fixed_tile else (0, 2, 4, 6, 8, 12, 16)):
         for r in runs:
             r.set_tile(tile)
         b = cp.Batch(runs, graph=True)
@@ -180,7 +181,7 @@ def batched_leg(cp, n_runs, steps, seq_steps, seed):
     # the runs continue on the plain path, one after another, from the time
     # the batch reached (t accumulated as t += dt, like the device clock)
     t_next = []
-    for r in runs:
+    for r in (runs if seq_steps > 0 else ()):
         t = 0.0
         for _ in range(1 + done):
             t += r.p.dt
@@ -197,14 +198,14 @@ def batched_leg(cp, n_runs, steps, seq_steps, seed):
     for r in runs:
         r.close()
     rs_b = n_runs * steps / wall_b
-    rs_s = n_runs * seq_steps / wall_s
+    rs_s = n_runs * seq_steps / wall_s if seq_steps > 0 else None
     return {"runs": n_runs, "particles": int(npart), "steps": steps,
             "tile_cells": tile, "tile_tuning_ms_per_step": tune,
             "ms_per_batched_step": 1e3 * wall_b / steps,
             "run_steps_per_s": rs_b, "particle_steps_per_s": npart * steps / wall_b,
             "sequential_plain_run_steps_per_s": rs_s,
             "sequential_plain_steps_per_run": seq_steps,
-            "speedup_vs_sequential_plain": rs_b / rs_s,
+            "speedup_vs_sequential_plain": rs_b / rs_s if rs_s else None,
             "workload": "256 runs, ncx~U{64..512}, ncy~U{32..256}, ppc~U{20..100}, "
                         "phi_p~U[-10,2], mi/me 100, Te/Ti 1, Jacobi FP64 residual 1e-6 "
                         "(max 500 sweeps), one HIP graph per batched step"}
@@ -336,6 +337,10 @@ def main():
                     help="cells per wave strip in the push passes (1..16); skips the tuning")
     ap.add_argument("--batched-runs", type=int, default=256)
     ap.add_argument("--batched-steps", type=int, default=10)
+    ap.add_argument("--batched-only", action="store_true",
+                    help="run only the batched leg (tile from --tile, else tuned; "
+                         "sequential baseline from --batched-seq-steps) and print its JSON")
+    ap.add_argument("--batched-seq-steps", type=int, default=3)
     ap.add_argument("--serve-requests", type=int, default=12)
     ap.add_argument("--serve-rate", type=float, default=50.0)
     ap.add_argument("--serve-max-steps", type=int, default=2000)
@@ -376,6 +381,11 @@ def main():
 
     import cupic_amd as cp
     name, cus = cp.init_dev(0)
+    if a.batched_only:
+        res = batched_leg(cp, a.batched_runs, a.batched_steps, a.batched_seq_steps,
+                          20261203, fixed_tile=a.tile)
+        print(json.dumps({"batched": res}), flush=True)
+        return
     cfg = CONFIGS[a.config]
     p = cp.params(**cfg, jacobi_mode=cp.JACOBI_RESIDUAL, jacobi_tol=1e-6,
                   jacobi_tol_init=1e-6, jacobi_max_sweeps=500, jacobi_check_every=50)
@@ -512,7 +522,8 @@ def main():
         out["plain_direct_poisson"] = direct_leg(cp, cfg, a.steps, a.warmup, tile)
         print(f"# direct: {out['plain_direct_poisson']}", file=sys.stderr, flush=True)
     if a.batched_runs > 0:
-        out["batched"] = batched_leg(cp, a.batched_runs, a.batched_steps, 3, 20261203)
+        out["batched"] = batched_leg(cp, a.batched_runs, a.batched_steps,
+                                     a.batched_seq_steps, 20261203)
         print(f"# batched: {out['batched']}", file=sys.stderr, flush=True)
     if a.serve_requests > 0:
         out["serving"] = serving_leg(cp, a.serve_requests, a.serve_rate, a.serve_max_steps,
