@@ -160,6 +160,7 @@ struct Target {
     cupic_run *timed;     // non-null: record timing events for this run
     hipStream_t side = nullptr;   // injection clock runs here when set
     hipEvent_t fork = nullptr, join = nullptr;
+    bool seg_stayers = false;     // scatter instance for low particles per cell
     Launch L(Shape s) const
     {
         return Launch{rds, plan + (size_t)s * (nruns + 1), nruns, nblk[s], stream};
@@ -176,6 +177,7 @@ Target target_of(cupic_run *r)
     T.side = r->side;
     T.fork = r->ev_fork;
     T.join = r->ev_join;
+    T.seg_stayers = (int64_t)r->p.per_row < (int64_t)kSegStayersPpc * r->p.ncx;
     return T;
 }
 
@@ -345,7 +347,7 @@ void seq_cc(const Target &T, int use_host_t, double t, int advance_t, bool clock
     }
     {
         TimeScope ts(T, kTScatter);
-        launch_scatter(T.L(kShapeTasks));
+        launch_scatter(T.L(kShapeTasks), T.seg_stayers);
     }
     {
         TimeScope ts(T, kTFinish);
@@ -1444,6 +1446,14 @@ Target batch_target(cupic_batch *b)
     T.side = b->side;
     T.fork = b->ev_fork;
     T.join = b->ev_join;
+    // one scatter launch serves every run: the instance follows the
+    // particles per cell over the whole batch
+    int64_t part = 0, cells = 0;
+    for (cupic_run *r : b->runs) {
+        part += (int64_t)r->p.per_row * r->p.ncy;
+        cells += (int64_t)r->p.ncx * r->p.ncy;
+    }
+    T.seg_stayers = part < (int64_t)kSegStayersPpc * cells;
     return T;
 }
 

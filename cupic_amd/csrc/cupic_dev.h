@@ -30,6 +30,10 @@ namespace cupic {
 constexpr int kWave = 64;
 constexpr int kBlock = 256;
 constexpr int kCmax = 16;           // max cells per wave task (tile_cells)
+// Below this many particles per cell the scatter sums the stayers of a chunk
+// that spans cells with one segmented scan (most chunks span cells there);
+// above it, per cell as the wave leaves the cell.
+constexpr int kSegStayersPpc = 128;
 constexpr int kTileW = kCmax + 3;   // deposit tile width (node columns)
 constexpr int kCountCells = 32;     // cells per wave strip of the counting pass (any tile)
 constexpr int kScanCells = 1024;    // cells per scan block

@@ -66,7 +66,8 @@ void launch_init_sort(const Launch &L);
 void launch_deposit(const Launch &L);
 void launch_half_kick(const Launch &L);
 void launch_push_rank(const Launch &L);
-void launch_scatter(const Launch &L);
+// seg_stayers: the instance for fewer than kSegStayersPpc particles per cell
+void launch_scatter(const Launch &L, bool seg_stayers);
 
 // boundary conditions and sort
 void launch_clock(const Launch &L, int use_host_t, double t_host);

@@ -635,6 +635,7 @@ __device__ __forceinline__ void count_task(const RunDev &R, RunState *st, const 
 // particle within (old cell, direction) with each counter started at its
 // slot base, stores the particle and its hint at the sorted position and
 // deposits its charge.
+template <bool kSegStayers>
 __device__ __forceinline__ void scatter_task(const RunDev &R, RunState *st, const TaskInfo &T,
                                              int *cntw, float2 *Et, unsigned long long *tile,
                                              int lane)
@@ -782,6 +783,22 @@ __device__ __forceinline__ void scatter_task(const RunDev &R, RunState *st, cons
             }
             if (stay) { acc1 += v1; acc2 += v2; }
             acc_n += ns;
+        } else if (kSegStayers) {
+            // a chunk that spans cells: the running sums are flushed, and
+            // one segmented scan gives every cell segment its stayer sums,
+            // which the segment's last lane adds to its cell
+            if (acc_cl >= 0) flush_stayers(tile, TW, acc_cl, acc1, acc2, sgn, lane);
+            acc1 = 0ull;
+            acc2 = 0ull;
+            acc_cl = -1;
+            acc_n = 0;
+            const int sstart = valid ? 63 - __clzll((long long)(heads & upto)) : lane;
+            const unsigned long long s1 = seg_scan_u64(stay ? v1 : 0ull, lane, sstart);
+            const unsigned long long s2 = seg_scan_u64(stay ? v2 : 0ull, lane, sstart);
+            const bool tail =
+                valid && (lane == kWave - 1 || i + 1 >= p1 || ((heads >> (lane + 1)) & 1ull));
+            if (tail && cl >= 0 && cl < nc && (s2 >> 48) != 0ull)
+                tile_add(tile, TW, 1 * TW + (cl + 1), s1, s2, sgn);
         } else {
             unsigned long long rem = heads;
             while (rem) {
@@ -836,6 +853,7 @@ __global__ __launch_bounds__(kBlock) void k_push_rank(const RunDev *rds, const i
                threadIdx.x & (kWave - 1));
 }
 
+template <bool kSegStayers>
 __global__ __launch_bounds__(kBlock) void k_scatter(const RunDev *rds, const int *start,
                                                     int nruns)
 {
@@ -846,7 +864,7 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const RunDev *rds, const int
     TaskInfo T = task_of(R, local, R.C, R.tpr);
     if (!T.ok) return;
     int w = threadIdx.x >> 6;
-    scatter_task(R, st, T, cnts[w], etiles[w], tiles[w], threadIdx.x & (kWave - 1));
+    scatter_task<kSegStayers>(R, st, T, cnts[w], etiles[w], tiles[w], threadIdx.x & (kWave - 1));
 }
 
 // ---------------------------------------------------------------- launchers
@@ -863,6 +881,12 @@ void launch_init_sort(const Launch &L) { LAUNCH_K(k_init_sort, L); }
 void launch_deposit(const Launch &L) { LAUNCH_K(k_deposit, L); }
 void launch_half_kick(const Launch &L) { LAUNCH_K(k_half_kick, L); }
 void launch_push_rank(const Launch &L) { LAUNCH_K(k_push_rank, L); }
-void launch_scatter(const Launch &L) { LAUNCH_K(k_scatter, L); }
+void launch_scatter(const Launch &L, bool seg_stayers)
+{
+    if (seg_stayers)
+        LAUNCH_K(k_scatter<true>, L);
+    else
+        LAUNCH_K(k_scatter<false>, L);
+}
 
 }  // namespace cupic
