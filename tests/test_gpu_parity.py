@@ -322,6 +322,45 @@ def test_batch_graph_equals_plain_runs(dev, mode):
         assert np.array_equal(ra.mesh(cp.MESH_PHI), rm.mesh(cp.MESH_PHI))
 
 
+@pytest.mark.parametrize("shapes", [
+    # the batch holds 48 particles per cell: the low-ppc scatter instance
+    # runs the 200-ppc run, which alone takes the other one
+    [dict(ncx=32, ncy=16, ppc=200), dict(ncx=64, ncy=32, ppc=10, seed=7)],
+    # the batch holds 162 particles per cell: the 200-ppc instance runs the
+    # 10-ppc run, which alone takes the low-ppc one
+    [dict(ncx=64, ncy=32, ppc=200), dict(ncx=32, ncy=16, ppc=10, seed=7)],
+], ids=["low-ppc-instance", "high-ppc-instance"])
+def test_scatter_instances_agree(dev, shapes):
+    """The scatter's two instances (stayer sums per cell, or by a segmented
+    scan below 128 particles per cell) deposit the same integers: a batch
+    whose particles per cell pick the other instance than a member's own
+    gives bit-identical states and potential to the run stepped alone."""
+    alone, mixed = [], []
+    for kw in shapes:
+        p = small(jacobi_max_sweeps=100, **kw)
+        say("shape", kw)
+        for lst in (alone, mixed):
+            r = cp.Run(p)
+            r.init_sim()
+            lst.append(r)
+    for r in alone:
+        t = r.p.dt
+        for _ in range(6):
+            r.step(t)
+            t += r.p.dt
+    b = cp.Batch(mixed, graph=True)
+    b.step(6)
+    b.close()
+    for ra, rm in zip(alone, mixed):
+        for s in (0, 1):
+            for u, v in zip(ra.particles(s), rm.particles(s)):
+                assert np.array_equal(u, v)
+        assert np.array_equal(ra.mesh(cp.MESH_PHI), rm.mesh(cp.MESH_PHI))
+        assert np.array_equal(ra.mesh(cp.MESH_RHO), rm.mesh(cp.MESH_RHO))
+    for r in alone + mixed:
+        r.close()
+
+
 def test_tile_does_not_change_results(dev):
     """The push passes' strip width (cells per wave) changes speed only:
     states, offsets, charge and tallies are bit-identical for every tile."""
