@@ -90,11 +90,83 @@ class Point(C.Structure):
 POINT_QUEUED, POINT_RUNNING, POINT_CONVERGED, POINT_CAPPED, POINT_FAILED = 0, 1, 2, 3, 4
 
 
+def _is_product_library(path):
+    """A mapped shared object that carries this library's C-ABI: its name
+    starts with libcupic and its dynamic string table holds cupic_init_dev
+    (other libraries of the repository export no cupic_ symbols)."""
+    base = os.path.basename(path)
+    if not base.startswith("libcupic") or ".so" not in base:
+        return False
+    import mmap
+    fname = path[:-len(" (deleted)")] if path.endswith(" (deleted)") else path
+    try:
+        with open(fname, "rb") as f, mmap.mmap(f.fileno(), 0, access=mmap.ACCESS_READ) as m:
+            return m.find(b"\0cupic_init_dev\0") >= 0
+    except (OSError, ValueError):
+        return True   # unreadable now (replaced or removed): count it as a copy
+
+
+_OWN_ID = None   # (st_dev, st_ino) of the file this package mapped
+
+
+def other_library_copies():
+    """Copies of libcupic.so mapped into this process other than the one
+    this package loaded (or, before loading, other than LIB_PATH): a file at
+    another path, or another file at this path, read from /proc/self/maps."""
+    own_id = _OWN_ID
+    if own_id is None:
+        try:
+            own = os.stat(LIB_PATH)
+            own_id = (own.st_dev, own.st_ino)
+        except OSError:
+            own_id = None
+    others = set()
+    try:
+        with open("/proc/self/maps") as f:
+            for line in f:
+                parts = line.split(None, 5)
+                if len(parts) < 6:
+                    continue
+                path = parts[5].strip()
+                if not path.startswith("/") or not _is_product_library(path):
+                    continue
+                dev = tuple(int(x, 16) for x in parts[3].split(":"))
+                ino = int(parts[4])
+                # /proc reports the device as major:minor; compare the inode
+                # and the path, since st_dev packs major/minor differently
+                if own_id is not None and ino == own_id[1] and \
+                        os.major(own_id[0]) == dev[0] and os.minor(own_id[0]) == dev[1]:
+                    continue
+                others.add(path)
+    except OSError:
+        pass
+    return sorted(others)
+
+
+class LibraryCopyError(ImportError):
+    """A second copy of libcupic.so is mapped into the process."""
+
+
+def _single_copy_check():
+    """Refuses a process that holds a second copy of the library.  Two
+    copies register the same kernel names, code objects and static state
+    with the one HIP runtime of the process; that configuration is not
+    supported (it is where a GPU memory fault was once seen)."""
+    others = other_library_copies()
+    if others:
+        raise LibraryCopyError(f"another copy of libcupic is already mapped into this process "
+                          f"({', '.join(others)}); load one copy per process")
+
+
 def _load():
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is not built; run `python cupic_amd/build.py` "
                           "(there is no CPU fallback)")
+    global _OWN_ID
+    _single_copy_check()
+    st = os.stat(LIB_PATH)
     lib = C.CDLL(LIB_PATH)
+    _OWN_ID = (st.st_dev, st.st_ino)
     P = C.POINTER(Params)
     vp = C.c_void_p
     i64p = C.POINTER(C.c_int64)
@@ -242,6 +314,7 @@ class Run:
     """One simulation: owns its device buffers and stream (src/main.cu:42-44)."""
 
     def __init__(self, p, capacity=0):
+        _single_copy_check()
         self.p = p.copy()
         self.h = C.c_void_p()
         _check(lib.cupic_run_create(C.byref(self.p), capacity, C.byref(self.h)))

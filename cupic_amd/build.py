@@ -25,7 +25,7 @@ HEADERS = ["cupic_dev.h", "cupic_launch.h"]
 ARCH = os.environ.get("CUPIC_OFFLOAD_ARCH", "gfx950")
 FLAGS = [
     "-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}",
-    "-ffp-contract=off",
+    "-ffp-contract=off", "-fvisibility=hidden",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-result",
     "-I", os.path.join(ROOT, "include"), "-I", CSRC,
@@ -62,7 +62,8 @@ def build(verbose=False):
         for out in ex.map(_run, jobs):
             if verbose and out.strip():
                 print(out)
-    _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-o", LIB + ".tmp"] + objs)
+    _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-o", LIB + ".tmp",
+          "-Wl,--version-script=" + os.path.join(CSRC, "libcupic.map")] + objs)
     os.replace(LIB + ".tmp", LIB)
     drv = os.path.join(CSRC, "cupic_main.cpp")
     if os.path.exists(drv):

@@ -458,6 +458,18 @@ Sizes sizes_of(const cupic_params *p, int64_t capacity_hint)
     return z;
 }
 
+// Slot indices, offsets and counters are int32, so a species buffer holds
+// at most kMaxCap slots; kernels address slots through float4 pointers
+// (64-bit byte offsets), never as 4 * slot int words.
+constexpr int64_t kMaxCap = 2000000000LL;
+
+int check_capacity(const Sizes &z, std::string *err)
+{
+    if (z.cap < z.n_init) return set_err(err, CUPIC_ERR_ARG, "capacity below initial load");
+    if (z.cap > kMaxCap) return set_err(err, CUPIC_ERR_ARG, "capacity too large (above 2e9 slots)");
+    return CUPIC_OK;
+}
+
 // [ncy+1][ncx] periodic -> reference layout [nny][nnx] with column nnx-1 = column 0
 template <class T>
 void to_reference(const std::vector<T> &in, int ncx, int ncy, T *out)
@@ -636,6 +648,7 @@ int cupic_run_footprint(const cupic_params *pin, int64_t capacity_hint, int64_t 
     int st = cupic_params_derive(&p);
     if (st) return st;
     Sizes z = sizes_of(&p, capacity_hint);
+    if ((st = check_capacity(z, nullptr))) return st;
     int64_t b = 0;
     b += 2 * 2 * 4 * z.cap * 4;                  // particles
     b += 2 * 2 * ((z.cap + 255) & ~(int64_t)255);  // push hints
@@ -662,8 +675,7 @@ int cupic_run_create(const cupic_params *pin, int64_t capacity_hint, cupic_run *
     int status = cupic_params_derive(&p);
     if (status) return status;
     Sizes z = sizes_of(&p, capacity_hint);
-    if (z.cap < z.n_init) return set_err(nullptr, CUPIC_ERR_ARG, "capacity below initial load");
-    if (z.cap > 2000000000LL) return set_err(nullptr, CUPIC_ERR_ARG, "capacity too large");
+    if ((status = check_capacity(z, nullptr))) return status;
     cupic_run *r = new cupic_run();
     r->p = p;
     RunDev &h = r->h;

@@ -296,8 +296,10 @@ __global__ __launch_bounds__(kBlock) void k_far_place(const RunDev *rds, const i
         int key = R.far[s][e].key;
         int slot = atomicAdd(&R.far_cur[s][key], 1);
         int pos = R.base[s][9 * R.ncells + key] + slot;
+        // the slot is addressed as a float4 (64-bit offset): 4 * pos as an
+        // int word index would overflow past 2^29 slots
         if ((unsigned)pos < (unsigned)R.cap)
-            ((int *)R.part[s][st->cur ^ 1])[4 * pos] = e;  // first word of the slot
+            reinterpret_cast<int *>(R.part[s][st->cur ^ 1] + pos)[0] = e;  // first word
     }
 }
 
@@ -316,7 +318,7 @@ __global__ __launch_bounds__(kBlock) void k_far_rank(const RunDev *rds, const in
         int key = far[e].key, idx = far[e].idx;
         int b = R.base[s][9 * R.ncells + key];
         int n = R.off[s][st->cur ^ 1][key + 1] - b;
-        const int *slots = (const int *)R.part[s][st->cur ^ 1];
+        const float4 *slots = R.part[s][st->cur ^ 1];
         // four slots per round trip: their entry indices, then the entries'
         // input indices (an index outside the list reads entry 0 and counts
         // for nothing)
@@ -324,7 +326,8 @@ __global__ __launch_bounds__(kBlock) void k_far_rank(const RunDev *rds, const in
         for (int u = 0; u < n; u += 4) {
             int q[4];
 #pragma unroll
-            for (int k = 0; k < 4; k++) q[k] = u + k < n ? slots[4 * (b + u + k)] : -1;
+            for (int k = 0; k < 4; k++)
+                q[k] = u + k < n ? reinterpret_cast<const int *>(slots + (b + u + k))[0] : -1;
 #pragma unroll
             for (int k = 0; k < 4; k++) {
                 const bool ok = (unsigned)q[k] < (unsigned)F;

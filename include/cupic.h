@@ -26,6 +26,11 @@
 extern "C" {
 #endif
 
+/* libcupic.so is built with hidden visibility: only the functions declared
+ * here are exported, so no kernel handle or internal helper of the library
+ * can bind to, or be bound by, another object in the process. */
+#pragma GCC visibility push(default)
+
 #define CUPIC_OK 0
 #define CUPIC_ERR_ARG (-1)        /* invalid argument or parameter           */
 #define CUPIC_ERR_HIP (-2)        /* HIP runtime error                       */
@@ -336,6 +341,8 @@ int cupic_timing_enable(cupic_run *r, int32_t on);
 int cupic_timing_select(cupic_run *r, uint32_t class_mask);
 int cupic_timing_read(cupic_run *r, double *ms, int64_t *launches, int32_t n);
 const char *cupic_timing_names(void);
+
+#pragma GCC visibility pop
 
 #ifdef __cplusplus
 }

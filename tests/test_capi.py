@@ -112,3 +112,68 @@ def test_input_file_n_ini_rewrite(tmp_path):
     with pytest.raises(cp.CupicError) as e:
         cp.input_file_set_n_ini(str(tmp_path / "missing"), 1)
     assert e.value.code == -7
+
+
+def _run_py(code, tmp_path):
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, "-c", code], cwd=ROOT, capture_output=True, text=True,
+                       timeout=120)
+    return r.returncode, r.stdout + r.stderr
+
+
+def test_second_library_copy_refused(tmp_path):
+    """A process that already maps a copy of libcupic.so (another file, for
+    example a variant build) cannot load the package's library beside it,
+    and a process that loaded the package refuses to create runs once a
+    copy appears.  Runs on the CPU: nothing reaches the GPU."""
+    import shutil
+    copy = tmp_path / "libcupic_variant.so"
+    shutil.copy(os.path.join(ROOT, "cupic_amd", "libcupic.so"), copy)
+    code = (f"import ctypes; ctypes.CDLL({str(copy)!r})\n"
+            "try:\n"
+            "    import cupic_amd\n"
+            "except ImportError as e:\n"
+            "    print('REFUSED', type(e).__name__, e)\n"
+            "else:\n"
+            "    print('LOADED')\n")
+    rc, out = _run_py(code, tmp_path)
+    assert rc == 0 and "REFUSED LibraryCopyError" in out and str(copy) in out, out
+    code = ("import cupic_amd as cp\n"
+            "assert cp.other_library_copies() == []\n"
+            f"import ctypes; ctypes.CDLL({str(copy)!r})\n"
+            f"assert cp.other_library_copies() == [{str(copy)!r}], cp.other_library_copies()\n"
+            "try:\n"
+            "    cp.Run(cp.params())\n"
+            "except cp.LibraryCopyError as e:\n"
+            "    print('REFUSED', e)\n")
+    rc, out = _run_py(code, tmp_path)
+    assert rc == 0 and "REFUSED" in out, out
+    # the oracle's library is test infrastructure, not a copy
+    code = ("from oracle import oracle as O; O.lib()\n"
+            "import cupic_amd as cp\n"
+            "print('COPIES', cp.other_library_copies())\n")
+    rc, out = _run_py(code, tmp_path)
+    assert rc == 0 and "COPIES []" in out, out
+
+
+def test_capacity_bound_and_large_runs():
+    """Slot indices and offsets are int32: a species buffer holds at most
+    2e9 slots, and runs well past 2^29 slots (where a 4 * slot int32 word
+    index would overflow) are accepted with their full footprint."""
+    import cupic_amd as cp
+    # 1024 x 2048 cells, 300 ppc: 629 M particles per species, cap ~ 708 M slots
+    p = cp.params(ncx=1024, ncy=2048, ppc=300)
+    n = p.per_row * p.ncy
+    assert n > 2 ** 29
+    b = cp.footprint(p)
+    assert b > 64 * n
+    with pytest.raises(cp.CupicError) as e:
+        cp.footprint(p, capacity=2_000_000_001)
+    assert e.value.code == -1 and "capacity" in str(e.value)
+    with pytest.raises(cp.CupicError):
+        cp.footprint(p, capacity=n - 1)       # below the initial load
+    assert cp.footprint(p, capacity=2_000_000_000) > 64 * 2_000_000_000
+    # the kernels address slots through float4 pointers, never as 4 * slot words
+    src = open(os.path.join(ROOT, "cupic_amd", "csrc", "k_cc.hip")).read()
+    assert "[4 * pos]" not in src and "[4 * (b + u + k)]" not in src

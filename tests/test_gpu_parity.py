@@ -76,8 +76,15 @@ def test_half_kick_bit_exact(dev):
                                 # smallest grid the library takes, the widest
                                 # one, and one with most cells empty
                                 dict(ncx=4, ncy=2, ppc=3), dict(ncx=4096, ncy=2, ppc=2),
-                                dict(ppc=1)],
-                         ids=["base", "odd", "quirks", "long_dt", "min_grid", "max_ncx", "sparse"])
+                                dict(ppc=1),
+                                # dense runs: at >= 128 ppc the scatter takes its
+                                # per-cell stayer instance, and most 64-particle
+                                # chunks lie in one cell; 70 ppc keeps the
+                                # segmented-scan instance with chunks over two cells
+                                dict(ncx=16, ncy=8, ppc=200), dict(ncx=20, ncy=10, ppc=130),
+                                dict(ncx=24, ncy=12, ppc=70, phi_p=-8.0)],
+                         ids=["base", "odd", "quirks", "long_dt", "min_grid", "max_ncx", "sparse",
+                              "ppc200", "ppc130", "ppc70"])
 def test_full_step_bit_exact(dev, kw):
     """deposit -> solve -> field -> move -> cc(t), compared with the FP32
     mirror given the run's own field and clock: sorted states, offsets,
