@@ -190,6 +190,7 @@ def _load():
         "cupic_adjust_leap_frog": [vp],
         "cupic_charge_deposition": [vp],
         "cupic_poisson_solver": [vp, C.c_double, i32p, dp],
+        "cupic_poisson_solver_noalloc": [P, vp, vp, vp, C.c_double, vp, i32p, dp],
         "cupic_field_solver": [vp],
         "cupic_particle_mover": [vp],
         "cupic_cc": [vp, C.c_double],
@@ -304,6 +305,23 @@ def footprint(p, capacity=0):
     b = C.c_int64()
     _check(lib.cupic_run_footprint(C.byref(p), capacity, C.byref(b)))
     return b.value
+
+
+def poisson_solver_noalloc(p, charge, phi, rho=None, tol=-1.0, stream=None):
+    """The Jacobi solve of Run.poisson_solver (modes 0 and 1) on caller-owned
+    device buffers (integer device addresses): charge int64 [ncy+1][ncx]
+    read, phi FP32 [ncy+1][ncx] in/out, rho FP32 out or None.  Allocates no
+    device memory: its temporaries live in a static device array of
+    NOALLOC_SCRATCH_BYTES per GPU, shared by all calls on that GPU.  Returns
+    (sweeps, residual) once the solve has finished."""
+    s = C.c_int32()
+    r = C.c_double()
+    _check(lib.cupic_poisson_solver_noalloc(C.byref(p), charge, phi, rho, tol, stream,
+                                            C.byref(s), C.byref(r)))
+    return s.value, r.value
+
+
+NOALLOC_SCRATCH_BYTES = 16 << 20
 
 
 def _f32(a):

@@ -309,6 +309,23 @@ void seq_solve(const Target &T, int use_args, int mode, double tol, int min_swee
     launch_solver_end(T.L(kShapeRun), c, nlaunch & 1);
 }
 
+}  // namespace
+
+namespace cupic {
+
+void seq_solve_one(const RunDev *d_rd, const int *d_plan, const int *nblk, hipStream_t stream,
+                   int mode, double tol, int min_sweeps, int max_sweeps, int check_every)
+{
+    Target T{d_rd, d_plan, nblk, 1, stream, nullptr};
+    seq_solve(T, 1, mode, tol, min_sweeps, max_sweeps, check_every);
+}
+
+int set_global_error(int code, const char *msg) { return set_err(nullptr, code, msg); }
+
+}  // namespace cupic
+
+namespace {
+
 void seq_rho(const Target &T)
 {
     TimeScope ts(T, kTDeposit);
@@ -499,7 +516,7 @@ int cupic_init_dev(int device, char *name, int name_len, int *compute_units)
         std::snprintf(name, (size_t)name_len, "%s (%s)", prop.name, prop.gcnArchName);
     }
     if (compute_units) *compute_units = prop.multiProcessorCount;
-    return CUPIC_OK;
+    return noalloc_setup(device);
 }
 
 int cupic_params_default(cupic_params *p)

@@ -243,6 +243,19 @@ __host__ __device__ inline int phi_alt(int a) { return a == 0 ? 1 : 0; }
 __host__ __device__ inline int phi_at(int a, int par) { return (par & 1) ? phi_alt(a) : a; }
 __host__ __device__ inline int phi_third(int a) { return 3 - a - phi_alt(a); }
 
+// Node charge a (int64, 2^-32 of a macro-particle charge) of row j -> rho
+// (volume ds^3, ds^3/2 on the two Dirichlet rows, the intent of
+// src/mesh.cu:242-268) and the Jacobi right-hand side ds^2 rho / eps0
+// (src/mesh.cu:327), 0 on the Dirichlet rows.
+__device__ __forceinline__ void charge_to_rhs(const RunDev &R, long long a, int j, float *rho,
+                                              double *b)
+{
+    const bool edge = (j == 0 || j == R.ncy);
+    const double q = (double)a;
+    *rho = (float)(q * (edge ? R.rho_edge : R.rho_interior));
+    *b = edge ? 0.0 : q * R.b_interior;
+}
+
 // Prologue of every kernel that takes (rds, start, nruns): the run R of this
 // workgroup, its block index `local` within the run, and the run state st.
 // start[0] is 0, so a launch for one run needs no load from start.

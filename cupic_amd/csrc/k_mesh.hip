@@ -37,10 +37,10 @@ __global__ __launch_bounds__(kBlock) void k_rho_convert(const RunDev *rds, const
     int j = k / R.ncx;
     long long a = (long long)R.acc[k];
     R.acc[k] = 0ull;
-    bool edge = (j == 0 || j == R.ncy);
-    double q = (double)a;
-    R.rho[k] = (float)(q * (edge ? R.rho_edge : R.rho_interior));
-    double bb = edge ? 0.0 : q * R.b_interior;
+    float rho;
+    double bb;
+    charge_to_rhs(R, a, j, &rho, &bb);
+    R.rho[k] = rho;
     R.b64[k] = bb;
     R.b32[k] = (float)bb;
 }

@@ -150,6 +150,35 @@ int cupic_charge_deposition(cupic_run *r);
 /* poisson_solver (src/mesh.cu:55-120).  tol < 0 uses the run's per-step
  * tolerance.  Returns sweeps run and the final residual (max |update|). */
 int cupic_poisson_solver(cupic_run *r, double tol, int32_t *sweeps, double *resid);
+
+/* Device scratch of cupic_poisson_solver_noalloc, held by every GPU the
+ * library runs on until the process ends. */
+#define CUPIC_NOALLOC_SCRATCH_BYTES (16u << 20)
+
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the Jacobi solve of cupic_poisson_solver, modes
+ * CUPIC_JACOBI_REFERENCE and CUPIC_JACOBI_RESIDUAL, on caller-owned device
+ * buffers, with every temporary in a static device array of
+ * CUPIC_NOALLOC_SCRATCH_BYTES per GPU.  No hipMalloc, hipFree or allocating
+ * library call is made.
+ *   p       grid, ds, eps0, jacobi_mode, sweep limits and check interval
+ *           (derived inside; p is not modified)
+ *   charge  int64 node charge [ncy+1][ncx] (periodic in x), 2^32 per unit
+ *           macro-particle charge, as CUPIC_MESH_CHARGE exports it; read only
+ *   phi     FP32 [ncy+1][ncx]: in, the starting iterate with the Dirichlet
+ *           rows 0 (probe) and ncy; out, the solution
+ *   rho     FP32 [ncy+1][ncx] charge density out, or NULL
+ *   tol     < 0 uses p->jacobi_tol
+ *   stream  hipStream_t to run on (NULL: the null stream); the call returns
+ *           once the solve has finished on it
+ * Grids whose temporaries exceed the scratch (about 20 bytes per node plus
+ * the per-sweep reduction slots: 1024 x 512 cells fit) are rejected with
+ * CUPIC_ERR_ARG.  Calls share their GPU's scratch, which keeps the last
+ * call's contents: calls that overlap on one GPU race. */
+int cupic_poisson_solver_noalloc(const cupic_params *p, const int64_t *charge, float *phi,
+                                 float *rho, double tol, void *stream, int32_t *sweeps,
+                                 double *resid);
 /* field_solver (src/mesh.cu:124-154): E = -grad phi */
 int cupic_field_solver(cupic_run *r);
 
