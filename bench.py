@@ -1,18 +1,24 @@
 """Full PIC cycle throughput on one MI355X.
 
-    python bench.py [--steps K] [--warmup W] [--config prod|small] [--cpu-seconds S]
+    python bench.py [--steps K] [--warmup W] [--config prod|small] [--solver direct|jacobi]
 
 Workload (default, "prod"): 1024x512 cells, ds 0.5, dt 0.05, 200 particles
 per cell per species (2 x 104,857,600), mi/me 1836, Te/Ti 10, probe bias
--10 kTe/e, Jacobi with FP64 residual to 1e-6 (at most 500 sweeps), FP32
-particles and fields, int64 fixed-point charge.  Synthetic seeded initial
-load; the first steps after init are warm-up.  One step is one pass of the
-cycle deposit -> solve -> field -> move -> cc over both species.
+-10 kTe/e, FP32 particles and fields, int64 fixed-point charge, the Poisson
+system solved to an FP64 residual below 1e-6 every step, 5000 steps (the
+whole run).  The solve is the fixed point of the Jacobi iteration computed
+directly in FP64 (DFT along x, sine transform along y); Jacobi capped at
+500 sweeps cannot reach 1e-6 on this grid and drives the potential out of
+[phi_p, 0] within a few hundred steps, which the "jacobi_capped" leg
+measures.  Synthetic seeded initial load; the first steps after init are
+warm-up.  One step is one pass of the cycle deposit -> solve -> field ->
+move -> cc over both species.
 
 Prints one JSON line: particle-steps/s over the timed steps (inputs resident
 in HBM), the roofline of the dominant kernel (HIP events on the run's
-stream), the full-cycle algorithmic-byte roofline, and a CPU baseline (the
-oracle's OpenMP FP64 build on a bounded sample of the same workload).
+stream), the measured HBM traffic of a whole step, and CPU baselines (the
+oracle's OpenMP FP64 build: a bounded sample of the same workload, and
+the small configuration's 1000 steps).
 """
 import argparse
 import json
@@ -33,20 +39,45 @@ CONFIGS = {
 }
 
 
-def workload_name(cfg, p):
+def workload_name(cfg, p, solver):
+    solve = ("Poisson to an FP64 residual < 1e-6 each step: the Jacobi fixed point solved "
+             "directly in FP64" if solver == "direct" else
+             f"Jacobi FP64 residual {p.jacobi_tol:g} (max {p.jacobi_max_sweeps} sweeps)")
     return (f"{cfg['ncx']}x{cfg['ncy']} cells, {cfg['ppc']} ppc/species "
             f"(2x{p.per_row * p.ncy} particles), mi/me={cfg['mi']:g}, "
-            f"Te/Ti={1.0 / cfg['kti']:g}, phi_p={cfg['phi_p']:g} kTe/e, "
-            f"Jacobi FP64 residual {p.jacobi_tol:g} (max {p.jacobi_max_sweeps} sweeps)")
+            f"Te/Ti={1.0 / cfg['kti']:g}, phi_p={cfg['phi_p']:g} kTe/e, {solve}")
+
+
+def physical_cores():
+    """Physical cores among the CPUs this process may run on, from
+    /proc/cpuinfo (distinct (package, core) pairs), and that CPU count."""
+    cpus = sorted(os.sched_getaffinity(0))
+    pairs = set()
+    try:
+        cur = {}
+        with open("/proc/cpuinfo") as f:
+            for line in list(f) + [""]:
+                if not line.strip():
+                    if "processor" in cur and int(cur["processor"]) in cpus:
+                        pairs.add((cur.get("physical id", "0"), cur.get("core id", cur["processor"])))
+                    cur = {}
+                    continue
+                k, _, v = line.partition(":")
+                cur[k.strip()] = v.strip()
+    except OSError:
+        pass
+    return (len(pairs) or None), len(cpus)
 
 
 def cpu_baseline(cfg, min_seconds, max_steps):
     """The oracle's OpenMP FP64 cycle on the same workload (the CPU checker,
     never the measured path): full cycles after its own init, repeated until
-    min_seconds of stepping have passed (at least 2, at most max_steps)."""
+    min_seconds of stepping have passed (at least 2, at most max_steps).  The
+    oracle solves Poisson with FP64 Jacobi sweeps (500 at most per step)."""
     from oracle import oracle as O
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
     os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+    phys, ncpu = physical_cores()
     p = O.make_params(**cfg)
     sim = O.Sim(p, omp=True)
     sim.set_solver(1, 1e-6, 1e-6, 0, 500, 50)
@@ -63,63 +94,115 @@ def cpu_baseline(cfg, min_seconds, max_steps):
     el = time.perf_counter() - t0
     del sim
     return {"value": n / el, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "physical_cores": phys, "cpus_available": ncpu, "steps": steps,
+            "ms_per_step": 1e3 * el / steps,
             "sample": f"{steps} full cycles of the same workload after init "
-                      f"(FP64, OpenMP, {threads} threads, Jacobi capped at 500 sweeps), "
-                      f"{el:.1f} s"}
+                      f"(FP64, OpenMP, {threads} threads on {phys} physical cores, Jacobi "
+                      f"capped at 500 sweeps), {el:.1f} s"}
+
+
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "cycle_1024x512_200ppc_hbm_traffic.md")
 
 
 def profiled_traffic(kernel):
-    """HBM bytes per particle of a kernel, from the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE summary (copy-calibrated), or None."""
-    path = os.path.join(ROOT, "profiles", "scatter_push_1024x512_200ppc_hbm_traffic.md")
+    """HBM bytes per particle of a kernel ("step": all kernels of one step),
+    from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
+    (copy-calibrated), or None."""
     try:
-        with open(path) as f:
+        with open(TRAFFIC_PROFILE) as f:
             for line in f:
                 w = line.split()
                 if len(w) == 3 and w[0] == "traffic_bytes_per_particle" and w[1] == kernel:
-                    return float(w[2]), os.path.relpath(path, ROOT)
+                    return float(w[2]), os.path.relpath(TRAFFIC_PROFILE, ROOT)
     except OSError:
         pass
     return None, None
 
-def direct_leg(cp, cfg, steps, warmup, tile):
-    """The same cycle with the Poisson system solved directly in FP64 (the
-    converged fixed point of the Jacobi iteration) instead of 500 sweeps."""
-    p = cp.params(**cfg, jacobi_mode=cp.POISSON_DIRECT, tile_cells=tile)
+def run_params(cp, cfg, solver, tile=0):
+    mode = cp.POISSON_DIRECT if solver == "direct" else cp.JACOBI_RESIDUAL
+    return cp.params(**cfg, jacobi_mode=mode, jacobi_tol=1e-6, jacobi_tol_init=1e-6,
+                     jacobi_max_sweeps=500, jacobi_check_every=50, tile_cells=tile)
+
+
+def end_state(cp, run):
+    """Solver residual, potential range, probe and sort statistics of a run
+    (read after its timed steps)."""
+    sweeps, resid = run.last_solve()
+    phi = run.mesh(cp.MESH_PHI)
+    tally = run.probe_tally()
+    return {"last_solve_sweeps": sweeps, "last_solve_fp64_residual": resid,
+            "phi_min": float(phi.min()), "phi_max": float(phi.max()),
+            "phi_p": run.p.phi_p,
+            "particles": [run.number_of_particles(0), run.number_of_particles(1)],
+            "probe_absorbed_total": [int(v) for v in tally["total_probe"]],
+            "probe_charge_total": tally["charge"]}
+
+
+def jacobi_capped_leg(cp, cfg, steps, tile):
+    """The same workload with FP32 Jacobi capped at 500 sweeps per step (FP64
+    residual checked every 50): its throughput over `steps` steps from init,
+    and where the potential has gone by then."""
+    p = run_params(cp, cfg, "jacobi", tile)
     run = cp.Run(p)
     run.init_sim()
     t = p.dt
-    for _ in range(warmup):
+    run.step(t)
+    t += p.dt
+    run.synchronize()
+    c0 = run.counters()
+    t0 = time.perf_counter()
+    for _ in range(steps - 1):
         run.step(t)
         t += p.dt
     run.synchronize()
-    n0 = run.number_of_particles(0) + run.number_of_particles(1)
-    run.timing(True, classes=["solve"])
-    run.timing_read()
+    wall = time.perf_counter() - t0
+    c1 = run.counters()
+    out = {"steps": steps, "ms_per_step": 1e3 * wall / (steps - 1),
+           "value": (c1["particle_steps"] - c0["particle_steps"]) / wall,
+           "unit": "particle-steps/s", "far_movers_total": c1["far"]}
+    try:
+        out.update(end_state(cp, run))
+    except cp.CupicError as e:
+        out["error"] = str(e)
+    out["note"] = ("500 sweeps remove about 1 % of the long-wavelength error per step on "
+                   "513 rows, so the FP64 residual stays far above 1e-6 and the potential "
+                   "drifts out of [phi_p, 0]: not the physics of the configuration")
+    run.close()
+    return out
+
+
+def small_leg(cp, steps):
+    """The small configuration (128x64 cells, 20 ppc, mi/me 100, Te/Ti 1,
+    phi_p -5, Jacobi FP64 residual 1e-6 with at most 500 sweeps) for its
+    whole run on the GPU."""
+    p = run_params(cp, CONFIGS["small"], "jacobi")
+    run = cp.Run(p)
+    run.init_sim()
+    run.synchronize()
+    c0 = run.counters()
+    t = p.dt
     t0 = time.perf_counter()
     for _ in range(steps):
         run.step(t)
         t += p.dt
     run.synchronize()
     wall = time.perf_counter() - t0
-    kern = run.timing_read()
-    n1 = run.number_of_particles(0) + run.number_of_particles(1)
-    _, resid = run.last_solve()
+    c1 = run.counters()
+    sw, _ = run.solver_totals()
+    out = {"steps": steps, "ms_per_step": 1e3 * wall / steps,
+           "value": (c1["particle_steps"] - c0["particle_steps"]) / wall,
+           "unit": "particle-steps/s", "jacobi_sweeps_per_step": sw / (steps + 1)}
+    out.update(end_state(cp, run))
     run.close()
-    solve_ms = kern["solve"][0] / max(kern["solve"][1], 1)
-    # the solve's four FP64 products: 2 x (m x ncx x ncx) for the DFT along x
-    # and 2 x (m x m x ncx) for the sine transform along y, m = ncy - 1
-    m, nx = p.ncy - 1, p.ncx
-    flops = 2.0 * (2.0 * m * nx * nx + 2.0 * m * m * nx)
-    achieved = flops / (solve_ms * 1e-3) / 1e12 if solve_ms > 0 else 0.0
-    return {"value": 0.5 * (n0 + n1) * steps / wall, "unit": "particle-steps/s",
-            "ms_per_step": 1e3 * wall / steps, "poisson_fp64_residual": resid,
-            "solve_ms_per_launch": solve_ms,
-            "solve_roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_MFMA_PEAK,
-                               "unit": "TFLOP/s", "frac": achieved / FP64_MFMA_PEAK,
-                               "flops_per_solve": flops,
-                               "note": "whole solve (four products, slice sums, FP64 "
-                                       "residual) against the dense FP64 matrix peak"}}
+    return out
+
+
+def cpu_small(steps):
+    """The oracle's OpenMP FP64 cycle on the small configuration, all steps."""
+    r = cpu_baseline(CONFIGS["small"], 1e9, steps)
+    r["sample"] = (f"the whole small-configuration run, {steps} steps after init (FP64, "
+                   f"OpenMP, {r['cores']} threads, Jacobi FP64 residual 1e-6, max 500 sweeps)")
+    return r
 
 
 def draw_batched(n_runs, seed):
@@ -324,17 +407,23 @@ def serving_leg(cp, n_req, rate, max_steps, seed):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=5000,
+                    help="timed steps (default: the configuration's whole 5000-step run)")
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="prod", choices=sorted(CONFIGS))
+    ap.add_argument("--solver", default=None, choices=["direct", "jacobi"],
+                    help="Poisson solve of the headline run (prod: direct, small: jacobi)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--cpu-max-steps", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-timing", action="store_true")
-    ap.add_argument("--no-direct", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--tile", type=int, default=0,
                     help="cells per wave strip in the push passes (1..16); skips the tuning")
+    ap.add_argument("--jacobi-steps", type=int, default=300,
+                    help="steps of the capped-Jacobi leg (0: skip)")
+    ap.add_argument("--small-steps", type=int, default=1000,
+                    help="steps of the small-configuration leg, GPU and CPU (0: skip)")
     ap.add_argument("--batched-runs", type=int, default=256)
     ap.add_argument("--batched-steps", type=int, default=10)
     ap.add_argument("--batched-only", action="store_true",
@@ -344,52 +433,35 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=12)
     ap.add_argument("--serve-rate", type=float, default=50.0)
     ap.add_argument("--serve-max-steps", type=int, default=2000)
-    ap.add_argument("--copy-probe-mib", type=int, default=0,
-                    help="first copy this many MiB device-to-device with torch: a plain copy "
-                         "kernel of known size that scales profiler byte counters")
+    ap.add_argument("--copy-probe-mib", type=int, default=1024,
+                    help="size of the plain copy/read/write kernels run first: the achievable "
+                         "HBM bandwidth, and the dispatch that scales profiler byte counters "
+                         "(0: skip)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: a single run of this path lives on one device "
               "(multi-GPU sharding is a design note, not part of this build)", file=sys.stderr)
         sys.exit(2)
-
-    if a.copy_probe_mib > 0:
-        import torch
-        n = a.copy_probe_mib * (1 << 20) // 4
-        src = torch.ones(n, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        torch.cuda.synchronize()
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        print(f"# copy probe: {a.copy_probe_mib} MiB read + {a.copy_probe_mib} MiB written",
-              file=sys.stderr, flush=True)
-        # achievable HBM bandwidth of plain streaming kernels, for reference
-        nb = n * 4
-        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
-        for label, fn, nbytes in (("copy", lambda: dst.copy_(src), 2 * nb),
-                                  ("read (sum)", lambda: src.sum(), nb),
-                                  ("write (fill)", lambda: dst.fill_(2.0), nb)):
-            fn()
-            ev[0].record()
-            for _ in range(5):
-                fn()
-            ev[1].record()
-            torch.cuda.synchronize()
-            ms = ev[0].elapsed_time(ev[1]) / 5
-            print(f"# {label}: {nbytes / ms / 1e6:.0f} GB/s", file=sys.stderr, flush=True)
-        del src, dst
+    solver = a.solver or ("direct" if a.config == "prod" else "jacobi")
 
     import cupic_amd as cp
     name, cus = cp.init_dev(0)
+    hbm_probe = {}
+    if a.copy_probe_mib > 0:
+        nb = a.copy_probe_mib << 20
+        for kind in ("copy", "read", "write"):
+            ms, rate = cp.bandwidth_probe(nb, kind)
+            hbm_probe[kind + "_GBps"] = rate / 1e9
+        print(f"# plain streaming kernels over {a.copy_probe_mib} MiB: {hbm_probe}",
+              file=sys.stderr, flush=True)
     if a.batched_only:
         res = batched_leg(cp, a.batched_runs, a.batched_steps, a.batched_seq_steps,
                           20261203, fixed_tile=a.tile)
         print(json.dumps({"batched": res}), flush=True)
         return
     cfg = CONFIGS[a.config]
-    p = cp.params(**cfg, jacobi_mode=cp.JACOBI_RESIDUAL, jacobi_tol=1e-6,
-                  jacobi_tol_init=1e-6, jacobi_max_sweeps=500, jacobi_check_every=50)
-    print(f"# {name}, {cus} CUs; {workload_name(cfg, p)}", file=sys.stderr, flush=True)
+    p = run_params(cp, cfg, solver)
+    print(f"# {name}, {cus} CUs; {workload_name(cfg, p, solver)}", file=sys.stderr, flush=True)
     run = cp.Run(p)
     t0 = time.perf_counter()
     run.init_sim()
@@ -407,8 +479,8 @@ def main():
         t, tile, _, ttimes = run.autotune(t, steps=2)
         print(f"# autotuned tile {tile}: {ttimes}", file=sys.stderr, flush=True)
     run.synchronize()
-    print("# warm-up done", file=sys.stderr, flush=True)
-    n0 = run.number_of_particles(0) + run.number_of_particles(1)
+    c0 = run.counters()
+    print(f"# warm-up done ({c0['steps']} steps)", file=sys.stderr, flush=True)
     sw0, _ = run.solver_totals()
     # The timed steps carry HIP events around the scatter only (the kernel
     # the roofline is quoted for); events around every kernel class cost
@@ -424,8 +496,9 @@ def main():
         t += p.dt
     run.synchronize()
     wall = time.perf_counter() - t0
-    n1 = run.number_of_particles(0) + run.number_of_particles(1)
+    c1 = run.counters()
     sw1, _ = run.solver_totals()
+    psteps = c1["particle_steps"] - c0["particle_steps"]
     kern = {}
     if not a.no_timing:
         kern_timed = run.timing_read()
@@ -438,14 +511,22 @@ def main():
         kern = run.timing_read()
         kern["scatter"] = kern_timed["scatter"]
     run.timing(False)
-    ps = 0.5 * (n0 + n1) * a.steps
-    value = ps / wall
+    # share of particles the counting pass pushed in full (hint missing or
+    # overturnable), from a few steps with the counter on
+    run.set_statistics(True)
+    ca = run.counters()
+    for _ in range(BREAKDOWN_STEPS):
+        run.step(t)
+        t += p.dt
+    cb = run.counters()
+    run.set_statistics(False)
+    und = (sum(cb["undecided"]) - sum(ca["undecided"])) / \
+        max(cb["particle_steps"] - ca["particle_steps"], 1)
+    value = psteps / wall
     ms_step = 1e3 * wall / a.steps
-    S = (sw1 - sw0) / a.steps
-    nodes = p.ncx * (p.ncy + 1)
-    npart = 0.5 * (n0 + n1)
-    cycle_bytes = 84.0 * npart + nodes * (32.0 + 12.0 * S)
-    tally = run.probe_tally()
+    npart = psteps / a.steps            # mean particles moved per step
+    state = end_state(cp, run)
+    far = [c1["far"][s] - c0["far"][s] for s in (0, 1)]
     out = {
         "metric": "particle-steps/s (full PIC cycle)",
         "value": value,
@@ -459,10 +540,14 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": workload_name(cfg, p), "grid": [p.ncx, p.ncy],
-                   "ppc": cfg["ppc"], "particles": int(n1), "jacobi_mode": "fp64-residual",
-                   "jacobi_sweeps_per_step": S, "tile_cells": tile or "auto",
-                   "device": name},
+        "config": {"workload": workload_name(cfg, p, solver), "grid": [p.ncx, p.ncy],
+                   "ppc": cfg["ppc"], "poisson": solver,
+                   "mean_particles_per_step": npart,
+                   "untimed_steps_before": c0["steps"],
+                   "jacobi_sweeps_per_step": (sw1 - sw0) / a.steps,
+                   "tile_cells": tile or "auto", "device": name, **state,
+                   "far_movers_per_step": [f / a.steps for f in far],
+                   "undecided_hint_share_end": und},
     }
     if kern:
         per = {k: (v[0] / v[1] if v[1] else 0.0) for k, v in kern.items()}
@@ -485,42 +570,25 @@ def main():
                            "traffic_source": src,
                            "bytes_per_particle": bytes_per_particle,
                            "avg_launch_ms": per[dom]}
-    out["cycle_roofline"] = {"bytes_per_step": cycle_bytes,
-                             "achieved_GBps": cycle_bytes / (wall / a.steps) / 1e9,
-                             "frac": cycle_bytes / (wall / a.steps) / HBM_PEAK,
-                             "formula": "84*(Ne+Ni) + nodes*(32 + 12*S)"}
-    design_bytes = 34.0 * npart + nodes * (32.0 + 12.0 * S)
-    out["cycle_roofline_design"] = {"bytes_per_step": design_bytes,
-                                    "frac": design_bytes / (wall / a.steps) / HBM_PEAK,
-                                    "formula": "34*(Ne+Ni) + nodes*(32 + 12*S): push_rank reads "
-                                               "the 1 B push hint, scatter reads 16 B and writes "
-                                               "16 B + the 1 B hint"}
-    if kern:
-        # the same bytes split between the particle kernels and the grid
-        # kernels, each over its own kernels' device time per step (the
-        # per-class times of the breakdown steps)
-        per = out["kernels_ms_per_launch"]
-        part_ms = sum(per[k] for k in ("push_rank", "inject", "scan", "scatter", "finish"))
-        grid_ms = sum(per[k] for k in ("deposit", "solve", "field"))
-        grid_bytes = nodes * (32.0 + 12.0 * S)
-        out["particle_kernels_roofline"] = {
-            "ms_per_step": part_ms,
-            "frac": 84.0 * npart / (part_ms * 1e-3) / HBM_PEAK,
-            "frac_design": 34.0 * npart / (part_ms * 1e-3) / HBM_PEAK,
-            "formula": "84*(Ne+Ni) (design: 34*(Ne+Ni)) over push_rank + inject + scan + "
-                       "scatter + finish"}
-        out["grid_kernels_roofline"] = {
-            "ms_per_step": grid_ms,
-            "frac": grid_bytes / (grid_ms * 1e-3) / HBM_PEAK,
-            "formula": "nodes*(32 + 12*S) over deposit + solve + field, against the HBM "
-                       "peak; the Jacobi grids stay in the 256 MB Infinity Cache, which "
-                       "serves most of these bytes"}
-    out["probe"] = {"total_absorbed": [int(v) for v in tally["total_probe"]],
-                    "charge_collected": tally["charge"]}
+    # measured HBM bytes of a whole step (every kernel, rocprofv3 counters,
+    # copy-calibrated) over the measured step time
+    tstep, src = profiled_traffic("step") if a.config == "prod" else (None, None)
+    if tstep:
+        bps = tstep * npart
+        out["step_hbm_traffic"] = {"bytes_per_step": bps,
+                                   "bytes_per_particle_step": tstep,
+                                   "achieved_GBps": bps / (ms_step * 1e-3) / 1e9,
+                                   "frac_of_peak": bps / (ms_step * 1e-3) / HBM_PEAK,
+                                   "source": src}
+    if hbm_probe:
+        out["hbm_probe"] = hbm_probe
     run.close()
-    if not a.no_direct:
-        out["plain_direct_poisson"] = direct_leg(cp, cfg, a.steps, a.warmup, tile)
-        print(f"# direct: {out['plain_direct_poisson']}", file=sys.stderr, flush=True)
+    if a.jacobi_steps > 0 and a.config == "prod" and solver == "direct":
+        out["jacobi_capped"] = jacobi_capped_leg(cp, cfg, a.jacobi_steps, tile)
+        print(f"# jacobi capped: {out['jacobi_capped']}", file=sys.stderr, flush=True)
+    if a.small_steps > 0:
+        out["small_config"] = small_leg(cp, a.small_steps)
+        print(f"# small config: {out['small_config']}", file=sys.stderr, flush=True)
     if a.batched_runs > 0:
         out["batched"] = batched_leg(cp, a.batched_runs, a.batched_steps,
                                      a.batched_seq_steps, 20261203)
@@ -531,6 +599,12 @@ def main():
     if not a.no_cpu and a.cpu_max_steps > 0:
         print("# cpu baseline ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(cfg, a.cpu_seconds, a.cpu_max_steps)
+        if a.small_steps > 0:
+            out["cpu_baseline_small_config"] = cpu_small(a.small_steps)
+            g = out.get("small_config", {}).get("value")
+            if g:
+                out["cpu_baseline_small_config"]["gpu_over_cpu"] = \
+                    g / out["cpu_baseline_small_config"]["value"]
     print(json.dumps(out), flush=True)
 
 

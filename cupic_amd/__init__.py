@@ -199,6 +199,8 @@ def _load():
         "cupic_probe_tally": [vp, i64p, i64p, i64p, i64p, dp],
         "cupic_last_solve": [vp, i32p, dp],
         "cupic_solver_totals": [vp, i64p, i64p],
+        "cupic_run_counters": [vp, i64p, i64p, i64p, i64p],
+        "cupic_run_set_statistics": [vp, C.c_int32],
         "cupic_get_particles": [vp, C.c_int, vp, vp, vp, vp, C.c_int64, i64p],
         "cupic_set_particles": [vp, C.c_int, vp, vp, vp, vp, C.c_int64],
         "cupic_get_offsets": [vp, C.c_int, vp],
@@ -232,6 +234,7 @@ def _load():
         "cupic_timing_enable": [vp, C.c_int32],
         "cupic_timing_select": [vp, C.c_uint32],
         "cupic_timing_read": [vp, dp, i64p, C.c_int32],
+        "cupic_bandwidth_probe": [C.c_int64, C.c_int32, C.c_int32, dp],
     }
     for name, args in sig.items():
         try:
@@ -322,6 +325,16 @@ def poisson_solver_noalloc(p, charge, phi, rho=None, tol=-1.0, stream=None):
 
 
 NOALLOC_SCRATCH_BYTES = 16 << 20
+
+
+def bandwidth_probe(nbytes, kind="copy", reps=5):
+    """Mean ms per launch of a plain streaming kernel over nbytes (copy:
+    nbytes read and nbytes written; read; write) and the rate in bytes/s."""
+    k = {"copy": 0, "read": 1, "write": 2}[kind]
+    ms = C.c_double()
+    _check(lib.cupic_bandwidth_probe(nbytes, k, reps, C.byref(ms)))
+    moved = nbytes * (2 if k == 0 else 1)
+    return ms.value, moved / (ms.value * 1e-3)
 
 
 def _f32(a):
@@ -430,6 +443,19 @@ class Run:
         a, b = C.c_int64(), C.c_int64()
         self._c(lib.cupic_solver_totals(self.h, C.byref(a), C.byref(b)))
         return a.value, b.value
+
+    def counters(self):
+        """Since init_sim: far movers and fully pushed (undecided-hint)
+        particles per species, steps, and particles moved summed over steps."""
+        far, und = (C.c_int64 * 2)(), (C.c_int64 * 2)()
+        n, ps = C.c_int64(), C.c_int64()
+        self._c(lib.cupic_run_counters(self.h, far, und, C.byref(n), C.byref(ps)))
+        return dict(far=[int(v) for v in far], undecided=[int(v) for v in und], steps=n.value,
+                    particle_steps=ps.value)
+
+    def set_statistics(self, on):
+        """Counts fully pushed (undecided-hint) particles while on; speed only."""
+        self._c(lib.cupic_run_set_statistics(self.h, 1 if on else 0))
 
     def particles(self, species):
         n = self.number_of_particles(species)

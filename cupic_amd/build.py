@@ -19,7 +19,7 @@ LIB = os.path.join(HERE, "libcupic.so")
 DRIVER = os.path.join(HERE, "cupic_main")
 
 SOURCES = ["k_mesh.hip", "k_particles.hip", "k_cc.hip", "k_direct.hip", "cupic_api.cpp",
-           "cupic_tune.cpp", "cupic_server.cpp", "cupic_noalloc.hip"]
+           "cupic_tune.cpp", "cupic_server.cpp", "cupic_noalloc.hip", "cupic_measure.hip"]
 HEADERS = ["cupic_dev.h", "cupic_launch.h"]
 
 ARCH = os.environ.get("CUPIC_OFFLOAD_ARCH", "gfx950")

@@ -658,6 +658,16 @@ int cupic_run_set_hint_mode(cupic_run *r, int32_t mode)
     return CUPIC_OK;
 }
 
+int cupic_run_set_statistics(cupic_run *r, int32_t on)
+{
+    if (!r) return set_err(nullptr, CUPIC_ERR_ARG, "null run");
+    if (r->moved) return set_err(&r->err, CUPIC_ERR_STATE, "set_statistics between mover and cc");
+    HIPTRY(&r->err, hipStreamSynchronize(r->stream));
+    r->h.count_stats = on ? 1 : 0;
+    HIPTRY(&r->err, hipMemcpy(r->d_rd, &r->h, sizeof(RunDev), hipMemcpyHostToDevice));
+    return CUPIC_OK;
+}
+
 int cupic_run_footprint(const cupic_params *pin, int64_t capacity_hint, int64_t *bytes)
 {
     if (!pin || !bytes) return set_err(nullptr, CUPIC_ERR_ARG, "null argument");
@@ -1033,6 +1043,21 @@ int cupic_solver_totals(cupic_run *r, int64_t *sweeps_total, int64_t *solves)
     int st = read_state(r, &s);
     if (sweeps_total) *sweeps_total = s.sweeps_total;
     if (solves) *solves = s.solves;
+    return st;
+}
+
+int cupic_run_counters(cupic_run *r, int64_t total_far[2], int64_t total_undecided[2],
+                       int64_t *steps, int64_t *particle_steps)
+{
+    if (!r) return set_err(nullptr, CUPIC_ERR_ARG, "null run");
+    RunState s;
+    int st = read_state(r, &s);
+    for (int k = 0; k < 2; k++) {
+        if (total_far) total_far[k] = (int64_t)s.total_far[k];
+        if (total_undecided) total_undecided[k] = (int64_t)s.total_undecided[k];
+    }
+    if (steps) *steps = s.step;
+    if (particle_steps) *particle_steps = (int64_t)s.particle_steps;
     return st;
 }
 

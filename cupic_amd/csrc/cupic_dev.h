@@ -100,6 +100,11 @@ struct RunState {
     unsigned long long probe[2], top[2];
     unsigned long long last_probe[2], last_top[2], last_injected[2];
     unsigned long long total_probe[2], total_top[2], total_injected[2];
+    // counting pass: particles pushed in full (hint missing or overturnable)
+    // this step, and running totals of those and of far movers
+    unsigned long long undecided[2];
+    unsigned long long total_undecided[2], total_far[2];
+    unsigned long long particle_steps;  // sum over steps of the particles moved
     double probe_charge_total;
     unsigned int err;               // error bits, see kErr*
 };
@@ -141,6 +146,8 @@ struct RunDev {
     unsigned long long seed, run_id;
     unsigned int quirks;
     int hint_mode;                  // kHintOff / kHintOn / kHintChecked
+    int count_stats;                // count fully pushed particles (one same-address
+                                    // atomic per wave of the counting pass: off by default)
     float4 *part[2][2];             // [species][buffer], (x, y, vx, vy) per particle
     unsigned char *hint[2][2];      // [species][buffer], push hint per particle slot
     int *off[2][2];                 // [species][buffer], ncells + 1

@@ -4,6 +4,7 @@
 // per-cell scan that turns direction counts into new offsets and slot
 // bases, the far-list scatter and the end-of-step bookkeeping (buffer flip,
 // probe tally).
+#include "cupic_clock.h"
 #include "cupic_launch.h"
 
 namespace cupic {
@@ -14,13 +15,19 @@ constexpr double kPi = 3.1415926535897932;  // src/stdh.h:26
 // Injection clock (src/cc.cu:35-36, 147, 179, 247): tin starts at the first
 // cc call as t + dtin; this step inserts in = 1 + int((t+dt-tin)/dtin)
 // particles when tin < t+dt, advancing tin by dtin once per particle.  The
-// recurrence runs serially in FP64 so the counts match the reference's
-// host loop bit for bit.  It reads only the clock, so a step can run it on a
+// counts and times must match the reference's FP64 host loop bit for bit;
+// the chain of adds is evaluated as the exact arithmetic progressions of
+// cupic_clock.h (one lane per species cuts it into segments, the block
+// writes the times).  It reads only the clock, so a step can run it on a
 // side stream while the field is solved.
+constexpr int kClockSegs = 64;
+
 __global__ __launch_bounds__(kBlock) void k_clock(const RunDev *rds, const int *start,
                                                   int nruns, int use_host_t, double t_host)
 {
     RUN_PROLOGUE();
+    __shared__ ClockSeg segs[2][kClockSegs];
+    __shared__ int nseg[2];
     if (threadIdx.x == 0) {
         if (use_host_t) st->t = t_host;
         if (!st->tin_started) {
@@ -30,28 +37,49 @@ __global__ __launch_bounds__(kBlock) void k_clock(const RunDev *rds, const int *
         }
     }
     __syncthreads();
-    int s = threadIdx.x;
-    if (s > 1) return;
-    double t = st->t;
-    double fpt = t + R.dt;
-    double tin = st->tin[s];
-    double dtin = R.sp[s].dtin;
-    int n = 0;
-    if (tin < fpt) n = 1 + (int)((fpt - tin) / dtin);
-    double *seq = R.inj_tin[s];
-    for (int k = 0; k < n; k++) {
-        if (k < R.max_inject) seq[k] = tin;
-        tin += dtin;
+    // lane 0 of waves 0 and 1: species 0 and 1
+    if ((threadIdx.x & (kWave - 1)) == 0 && threadIdx.x < 2 * kWave) {
+        const int s = threadIdx.x / kWave;
+        const double t = st->t;
+        const double fpt = t + R.dt;
+        const double tin = st->tin[s];
+        const double dtin = R.sp[s].dtin;
+        int n = 0;
+        if (tin < fpt) n = 1 + (int)((fpt - tin) / dtin);
+        const int nst = min(n, R.max_inject);  // times kept for the injection
+        double tend;
+        int ns = clock_segments(tin, dtin, nst, segs[s], kClockSegs, &tend);
+        if (ns < 0) {
+            // more segments than the list holds (ties at many binades, or a
+            // clock with a long way to go): plain adds for the rest
+            CUPIC_GLOBAL double *seq = gp(R.inj_tin[s]);
+            const ClockSeg &lastseg = segs[s][kClockSegs - 1];
+            for (int k = lastseg.k0 + lastseg.len; k < nst; k++) {
+                seq[k] = tend;
+                tend += dtin;
+            }
+            ns = kClockSegs;
+        }
+        for (int k = nst; k < n; k++) tend += dtin;
+        nseg[s] = ns;
+        st->tin[s] = tend;
+        int ne = n;
+        unsigned int err = 0;
+        if (ne > R.max_inject) { ne = R.max_inject; err |= kErrInjectCap; }
+        if (st->N[s] + ne > R.cap) { ne = max(0, R.cap - st->N[s]); err |= kErrCapacity; }
+        if (err) atomicOr(&st->err, err);
+        st->n_in[s] = ne;
+        st->ninj_first[s] = st->ninj[s];
+        st->ninj[s] += (unsigned long long)ne;
     }
-    st->tin[s] = tin;
-    int ne = n;
-    unsigned int err = 0;
-    if (ne > R.max_inject) { ne = R.max_inject; err |= kErrInjectCap; }
-    if (st->N[s] + ne > R.cap) { ne = max(0, R.cap - st->N[s]); err |= kErrCapacity; }
-    if (err) atomicOr(&st->err, err);
-    st->n_in[s] = ne;
-    st->ninj_first[s] = st->ninj[s];
-    st->ninj[s] += (unsigned long long)ne;
+    __syncthreads();
+    for (int s = 0; s < 2; s++) {
+        CUPIC_GLOBAL double *seq = gp(R.inj_tin[s]);
+        for (int q = 0; q < nseg[s]; q++) {
+            const ClockSeg g = segs[s][q];
+            for (int j = threadIdx.x; j < g.len; j += kBlock) seq[g.k0 + j] = clock_value(g, j);
+        }
+    }
 }
 
 // One injected particle (src/cc.cu:214-247), evaluated in FP64: x = U Lx,
@@ -390,6 +418,10 @@ __global__ __launch_bounds__(kBlock) void k_finish(const RunDev *rds, const int 
         st->total_probe[s] += st->probe[s];
         st->total_top[s] += st->top[s];
         st->total_injected[s] += (unsigned long long)st->n_in[s];
+        st->total_undecided[s] += st->undecided[s];
+        st->total_far[s] += (unsigned long long)st->far_n[s];
+        st->undecided[s] = 0;
+        st->particle_steps += (unsigned long long)st->N[s];
         dq += R.sp[s].q * (double)st->probe[s];
         st->probe[s] = 0;
         st->top[s] = 0;
@@ -417,12 +449,14 @@ __global__ __launch_bounds__(kBlock) void k_init_finish(const RunDev *rds, const
         st->probe[s] = st->top[s] = 0;
         st->last_probe[s] = st->last_top[s] = st->last_injected[s] = 0;
         st->total_probe[s] = st->total_top[s] = st->total_injected[s] = 0;
+        st->undecided[s] = st->total_undecided[s] = st->total_far[s] = 0;
         st->ninj[s] = 0;
         st->ninj_first[s] = 0;
         st->tin[s] = 0.0;
     }
     st->tin_started = 0;
     st->probe_charge_total = 0.0;
+    st->particle_steps = 0;
     st->sweeps_total = 0;
     st->solves = 0;
     st->step = 0;

@@ -464,7 +464,7 @@ __device__ __forceinline__ void count_task(const RunDev &R, RunState *st, const 
     for (int q = lane; q <= nc; q += kWave) bnd[q] = off[c0 + q];
     wave_lds_sync();
     const unsigned long long lt = (1ull << lane) - 1ull;
-    unsigned long long probe_n = 0, top_n = 0;
+    unsigned long long probe_n = 0, top_n = 0, full_n = 0;
     bool bad = false, wrong = false;
 
     // full push of queued slot i (lanes with ok false take no part)
@@ -570,6 +570,7 @@ __device__ __forceinline__ void count_task(const RunDev &R, RunState *st, const 
             const unsigned long long fq = __ballot(full);
             if (full) queue[(qh + qn + mbcnt64(fq)) & (kQueue - 1)] = i;
             qn += __popcll(fq);
+            full_n += __popcll(fq);
         }
         // at most kWave - 1 + 4 kWave entries: the ring never overfills
         while (qn >= kWave) {
@@ -626,6 +627,7 @@ __device__ __forceinline__ void count_task(const RunDev &R, RunState *st, const 
     if (lane == 0) {
         if (probe_n) atomicAdd(&st->probe[s], probe_n);
         if (top_n) atomicAdd(&st->top[s], top_n);
+        if (R.count_stats && full_n) atomicAdd(&st->undecided[s], full_n);
     }
     if (__ballot(bad) && lane == 0) atomicOr(&st->err, kErrSortInvariant);
     if (__ballot(wrong) && lane == 0) atomicOr(&st->err, kErrHint);

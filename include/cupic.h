@@ -209,6 +209,18 @@ int cupic_probe_tally(cupic_run *r, int64_t last_probe[2], int64_t last_top[2],
 int cupic_last_solve(cupic_run *r, int32_t *sweeps, double *resid);
 /* Jacobi sweeps and solves since init_sim */
 int cupic_solver_totals(cupic_run *r, int64_t *sweeps_total, int64_t *solves);
+/* Statistics since init_sim: per species, particles that moved more than
+ * one cell in a step (the far list) and particles the counting pass pushed
+ * in full because their push hint was missing or the field could overturn
+ * it (counted only while cupic_run_set_statistics is on); the steps taken
+ * and the particles moved summed over those steps. */
+int cupic_run_counters(cupic_run *r, int64_t total_far[2], int64_t total_undecided[2],
+                       int64_t *steps, int64_t *particle_steps);
+/* Turns the count of fully pushed particles on or off (off after
+ * cupic_run_create).  It costs one same-address atomic per wave of the
+ * counting pass, about 0.3 ms per step at 1024 x 512 cells; results never
+ * depend on it. */
+int cupic_run_set_statistics(cupic_run *r, int32_t on);
 int cupic_get_particles(cupic_run *r, int species, float *x, float *y, float *vx, float *vy,
                         int64_t capacity, int64_t *n);
 /* uploads a particle state (any order); it is cell-sorted on the device */
@@ -370,6 +382,13 @@ int cupic_timing_enable(cupic_run *r, int32_t on);
 int cupic_timing_select(cupic_run *r, uint32_t class_mask);
 int cupic_timing_read(cupic_run *r, double *ms, int64_t *launches, int32_t n);
 const char *cupic_timing_names(void);
+
+/* Plain streaming kernel of known traffic (kind 0: copy, bytes read and
+ * bytes written; 1: read; 2: write) over a fresh device buffer of `bytes`,
+ * timed over reps launches on the null stream: mean ms per launch.  The
+ * achievable HBM bandwidth, and the reference that scales profiler byte
+ * counters into bytes.  Allocates and frees its buffers. */
+int cupic_bandwidth_probe(int64_t bytes, int32_t kind, int32_t reps, double *ms);
 
 #pragma GCC visibility pop
 
