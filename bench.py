@@ -570,6 +570,19 @@ def main():
                            "traffic_source": src,
                            "bytes_per_particle": bytes_per_particle,
                            "avg_launch_ms": per[dom]}
+    if kern and solver == "direct":
+        # the direct solve's four FP64 products: 2 x (m x ncx x ncx) for the
+        # DFT along x and 2 x (m x m x ncx) for the sine transform along y,
+        # m = ncy - 1, against the dense FP64 matrix peak
+        solve_ms = out["kernels_ms_per_launch"]["solve"]
+        m, nx = p.ncy - 1, p.ncx
+        flops = 2.0 * (2.0 * m * nx * nx + 2.0 * m * m * nx)
+        ach = flops / (solve_ms * 1e-3) / 1e12 if solve_ms > 0 else 0.0
+        out["solve_roofline"] = {"bound": "mfma", "achieved": ach, "peak": FP64_MFMA_PEAK,
+                                 "unit": "TFLOP/s", "frac": ach / FP64_MFMA_PEAK,
+                                 "flops_per_solve": flops, "avg_launch_ms": solve_ms,
+                                 "note": "whole solve (four products, slice sums, FP64 "
+                                         "residual) against the dense FP64 matrix peak"}
     # measured HBM bytes of a whole step (every kernel, rocprofv3 counters,
     # copy-calibrated) over the measured step time
     tstep, src = profiled_traffic("step") if a.config == "prod" else (None, None)
