@@ -475,10 +475,13 @@ __device__ __forceinline__ Pushed push_one(const G &R, const float2 *__restrict_
 }
 
 // 16-bit-per-axis fixed-point weights; every particle carries 2^32 units.
+// Fractions lie in [0, 1], so the weights are at most 2^16; the mask (a
+// no-op on them) tells the compiler so, and products of weights become
+// full-rate 24-bit multiplies.
 __device__ __forceinline__ void fixed_weights(float fx, float fy, unsigned &wx, unsigned &wy)
 {
-    wx = (unsigned)rintf(fx * 65536.0f);
-    wy = (unsigned)rintf(fy * 65536.0f);
+    wx = (unsigned)rintf(fx * 65536.0f) & 0x1FFFFu;
+    wy = (unsigned)rintf(fy * 65536.0f) & 0x1FFFFu;
 }
 
 // ---------------------------------------------------------------- push hints

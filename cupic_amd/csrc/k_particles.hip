@@ -146,7 +146,7 @@ __global__ __launch_bounds__(kBlock) void k_init_sort(const RunDev *rds, const i
 //   c00 = 2^32 n - 2^16 (Swx + Swy) + Swxwy, c10 = 2^16 Swx - Swxwy,
 //   c01 = 2^16 Swy - Swxwy,                 c11 = Swxwy.
 __device__ __forceinline__ void wave_deposit(unsigned long long *tile, int TW, int dkey,
-                                             unsigned wx, unsigned wy, long long sgn, int lane)
+                                             unsigned wx, unsigned wy, int lane)
 {
     unsigned long long v1 = 0, v2 = 0;
     if (dkey >= 0) {
@@ -170,22 +170,24 @@ __device__ __forceinline__ void wave_deposit(unsigned long long *tile, int TW, i
             long long c10 = (sx << 16) - sxy;
             long long c01 = (sy << 16) - sxy;
             long long c00 = (n << 32) - ((sx + sy) << 16) + sxy;
-            atomicAdd(&tile[lk], (unsigned long long)(sgn * c00));
-            atomicAdd(&tile[lk + 1], (unsigned long long)(sgn * c10));
-            atomicAdd(&tile[lk + TW], (unsigned long long)(sgn * c01));
-            atomicAdd(&tile[lk + TW + 1], (unsigned long long)(sgn * c11));
+            atomicAdd(&tile[lk], (unsigned long long)c00);
+            atomicAdd(&tile[lk + 1], (unsigned long long)c10);
+            atomicAdd(&tile[lk + TW], (unsigned long long)c01);
+            atomicAdd(&tile[lk + TW + 1], (unsigned long long)c11);
         }
         pending &= ~m;
     }
 }
 
-// Tile rows j0-1 .. j0+2, columns col0-1 .. col0+nc+1 (wrapped) -> HBM.
+// Tile rows j0-1 .. j0+2, columns col0-1 .. col0+nc+1 (wrapped) -> HBM.  The
+// tile holds charge counts of one species; electrons (neg) enter negated.
 __device__ __forceinline__ void flush_tile(const RunDev &R, const unsigned long long *tile,
-                                           int TW, int j0, int col0, int lane)
+                                           int TW, int j0, int col0, int lane, bool neg)
 {
     for (int q = lane; q < 4 * TW; q += kWave) {
         unsigned long long v = tile[q];
         if (v == 0ull) continue;
+        if (neg) v = 0ull - v;
         int row = j0 - 1 + q / TW;
         int col = col0 - 1 + (q % TW);
         if (col < 0) col += R.ncx;
@@ -260,10 +262,10 @@ __global__ __launch_bounds__(kBlock) void k_deposit(const RunDev *rds, const int
                           (unsigned long long)(sgn * (long long)wx * (long long)wy));
             }
         }
-        wave_deposit(tile, TW, dkey, wx, wy, sgn, lane);
+        wave_deposit(tile, TW, dkey, wx, wy, lane);
     }
     wave_lds_sync();
-    flush_tile(R, tile, TW, T.j0, T.col0, lane);
+    flush_tile(R, tile, TW, T.j0, T.col0, lane, T.s == 0);
 }
 
 // Initial velocity stagger, src/init.cu:793-833: v -= dt/2 * qE/m.
@@ -345,28 +347,41 @@ __device__ __forceinline__ int dpp_prev_int(int v)
 // Packing: a1 = sum wx | sum wy << 32, a2 = sum wx*wy | count << 48; exact
 // while a group holds fewer than 65536 particles.
 __device__ __forceinline__ void tile_add(unsigned long long *tile, int TW, int key,
-                                         unsigned long long a1, unsigned long long a2,
-                                         long long sgn)
+                                         unsigned long long a1, unsigned long long a2)
 {
-    long long sx = (long long)(a1 & 0xFFFFFFFFull);
-    long long sy = (long long)(a1 >> 32);
-    long long n = (long long)(a2 >> 48);
-    long long sxy = (long long)(a2 & ((1ull << 48) - 1));
-    atomicAdd(&tile[key], (unsigned long long)(sgn * ((n << 32) - ((sx + sy) << 16) + sxy)));
-    atomicAdd(&tile[key + 1], (unsigned long long)(sgn * ((sx << 16) - sxy)));
-    atomicAdd(&tile[key + TW], (unsigned long long)(sgn * ((sy << 16) - sxy)));
-    atomicAdd(&tile[key + TW + 1], (unsigned long long)(sgn * sxy));
+    unsigned long long sx = a1 & 0xFFFFFFFFull;
+    unsigned long long sy = a1 >> 32;
+    unsigned long long n = a2 >> 48;
+    unsigned long long sxy = a2 & ((1ull << 48) - 1);
+    atomicAdd(&tile[key], (n << 32) - ((sx + sy) << 16) + sxy);
+    atomicAdd(&tile[key + 1], (sx << 16) - sxy);
+    atomicAdd(&tile[key + TW], (sy << 16) - sxy);
+    atomicAdd(&tile[key + TW + 1], sxy);
+}
+
+// One particle's charge: the four corner products of its 16-bit weights
+// (at most 2^16 each, so full-rate 24-bit multiplies) into tile cell key.
+__device__ __forceinline__ void tile_add_one(unsigned long long *tile, int TW, int key,
+                                             unsigned wx, unsigned wy)
+{
+    // the masks are no-ops (weights lie in [0, 2^16]) that keep every factor
+    // visibly below 2^24
+    const unsigned long long gx = (65536u - wx) & 0x1FFFFu, gy = (65536u - wy) & 0x1FFFFu;
+    atomicAdd(&tile[key], gx * gy);
+    atomicAdd(&tile[key + 1], (unsigned long long)wx * gy);
+    atomicAdd(&tile[key + TW], gx * (unsigned long long)wy);
+    atomicAdd(&tile[key + TW + 1], (unsigned long long)wx * (unsigned long long)wy);
 }
 
 // Reduces the per-lane stayer sums of old cell cl over the wave and adds
 // them to the tile cell of cl (stayers keep their cell).
 __device__ __forceinline__ void flush_stayers(unsigned long long *tile, int TW, int cl,
                                               unsigned long long a1, unsigned long long a2,
-                                              long long sgn, int lane)
+                                              int lane)
 {
     a1 = seg_scan_u64(a1, lane, 0);
     a2 = seg_scan_u64(a2, lane, 0);
-    if (lane == kWave - 1 && (a2 >> 48) != 0ull) tile_add(tile, TW, 1 * TW + (cl + 1), a1, a2, sgn);
+    if (lane == kWave - 1 && (a2 >> 48) != 0ull) tile_add(tile, TW, 1 * TW + (cl + 1), a1, a2);
 }
 
 // LDS tile of E on the two node rows of a strip: columns col0 .. col0+nc.
@@ -655,7 +670,6 @@ __device__ __forceinline__ void scatter_task(const RunDev &R, RunState *st, cons
     const int ncells = R.ncells;
     const float2 *E = R.E;
     const float qdtm = R.sp[s].qdtm;
-    const long long sgn = s == 0 ? -1 : 1;
     const int TW = nc + 3;
     load_E_tile(R, Et, j0, col0, nc, lane);
     {
@@ -745,11 +759,11 @@ __device__ __forceinline__ void scatter_task(const RunDev &R, RunState *st, cons
                 unsigned long long hi = after ? ((1ull << (__ffsll((long long)after) - 1)) - 1ull) : ~0ull;
                 unsigned long long seg = hi & ~((1ull << seg_start) - 1ull);
                 unsigned long long grp = mymask & seg;
-                rank = cntw[cl * 9 + dir] + __popcll(grp & lt);
+                rank = cntw[__mul24(cl, 9) + dir] + __popcll(grp & lt);
                 last = (grp & ~upto) == 0ull;
             }
             wave_lds_sync();
-            if (dir < 9 && last) cntw[cl * 9 + dir] = rank + 1;
+            if (dir < 9 && last) cntw[__mul24(cl, 9) + dir] = rank + 1;
         }
         wave_lds_sync();
         unsigned wx = 0, wy = 0;
@@ -777,7 +791,7 @@ __device__ __forceinline__ void scatter_task(const RunDev &R, RunState *st, cons
         if (one_cell) {
             const int ns = __popcll(stay_m);
             if (cl0 != acc_cl || acc_n + ns >= 60000) {
-                if (acc_cl >= 0) flush_stayers(tile, TW, acc_cl, acc1, acc2, sgn, lane);
+                if (acc_cl >= 0) flush_stayers(tile, TW, acc_cl, acc1, acc2, lane);
                 acc1 = 0ull;
                 acc2 = 0ull;
                 acc_cl = cl0;
@@ -789,7 +803,7 @@ __device__ __forceinline__ void scatter_task(const RunDev &R, RunState *st, cons
             // a chunk that spans cells: the running sums are flushed, and
             // one segmented scan gives every cell segment its stayer sums,
             // which the segment's last lane adds to its cell
-            if (acc_cl >= 0) flush_stayers(tile, TW, acc_cl, acc1, acc2, sgn, lane);
+            if (acc_cl >= 0) flush_stayers(tile, TW, acc_cl, acc1, acc2, lane);
             acc1 = 0ull;
             acc2 = 0ull;
             acc_cl = -1;
@@ -800,7 +814,7 @@ __device__ __forceinline__ void scatter_task(const RunDev &R, RunState *st, cons
             const bool tail =
                 valid && (lane == kWave - 1 || i + 1 >= p1 || ((heads >> (lane + 1)) & 1ull));
             if (tail && cl >= 0 && cl < nc && (s2 >> 48) != 0ull)
-                tile_add(tile, TW, 1 * TW + (cl + 1), s1, s2, sgn);
+                tile_add(tile, TW, 1 * TW + (cl + 1), s1, s2);
         } else {
             unsigned long long rem = heads;
             while (rem) {
@@ -813,7 +827,7 @@ __device__ __forceinline__ void scatter_task(const RunDev &R, RunState *st, cons
                 if (c < 0 || c >= nc) continue;  // outside the strip: no stayers
                 const int ns = __popcll(stay_m & M);
                 if (c != acc_cl || acc_n + ns >= 60000) {
-                    if (acc_cl >= 0) flush_stayers(tile, TW, acc_cl, acc1, acc2, sgn, lane);
+                    if (acc_cl >= 0) flush_stayers(tile, TW, acc_cl, acc1, acc2, lane);
                     acc1 = 0ull;
                     acc2 = 0ull;
                     acc_cl = c;
@@ -824,17 +838,17 @@ __device__ __forceinline__ void scatter_task(const RunDev &R, RunState *st, cons
             }
         }
         if (dir < 9 && !stay) {
-            int key = (o.njc - (j0 - 1)) * TW + (cl + o.di + 1);
-            tile_add(tile, TW, key, v1, v2, sgn);
+            int key = __mul24(o.njc - (j0 - 1), TW) + (cl + o.di + 1);
+            tile_add_one(tile, TW, key, wx, wy);
         }
     }
     if (pdest >= 0) {
         st4(POUT, pdest, prec);
         HOUT[pdest] = (unsigned char)phint;
     }
-    if (acc_cl >= 0) flush_stayers(tile, TW, acc_cl, acc1, acc2, sgn, lane);
+    if (acc_cl >= 0) flush_stayers(tile, TW, acc_cl, acc1, acc2, lane);
     wave_lds_sync();
-    flush_tile(R, tile, TW, j0, col0, lane);
+    flush_tile(R, tile, TW, j0, col0, lane, s == 0);
     if (__ballot(bad) && lane == 0) atomicOr(&st->err, kErrSortInvariant);
 }
 
