@@ -218,16 +218,19 @@ def draw_batched(n_runs, seed):
     return out
 
 
-def batched_leg(cp, n_runs, steps, seq_steps, seed, fixed_tile=0):
+def batched_leg(cp, n_runs, steps, seq_steps, seed, fixed_tile=0, solver="direct"):
     """One HIP-graph step advances all runs; baseline: the same runs stepped
     one after another through the plain path (cupic_step per run), skipped
     when seq_steps is 0.  The tile (cells per wave strip) is tuned in place
     on the batch itself: each candidate is set on every run and timed over 2
-    batched steps, unless fixed_tile names one."""
+    batched steps, unless fixed_tile names one.  The Poisson system is solved
+    to the FP64 residual 1e-6 (directly); Jacobi capped at 500 sweeps lets
+    the larger grids' potentials run away within a few hundred steps."""
     draws = draw_batched(n_runs, seed)
     runs = []
+    mode = cp.POISSON_DIRECT if solver == "direct" else cp.JACOBI_RESIDUAL
     for d in draws:
-        p = cp.params(**d, jacobi_mode=cp.JACOBI_RESIDUAL, jacobi_tol=1e-6,
+        p = cp.params(**d, jacobi_mode=mode, jacobi_tol=1e-6,
                       jacobi_tol_init=1e-6, jacobi_max_sweeps=500, jacobi_check_every=50)
         r = cp.Run(p)
         r.init_sim()
@@ -261,6 +264,14 @@ def batched_leg(cp, n_runs, steps, seq_steps, seed, fixed_tile=0):
     wall_b = time.perf_counter() - t0
     b.close()
     done += steps
+    # where the runs' potentials are after the window: runs whose potential
+    # left the range spanned by the bias and the grounded top by more than
+    # 10 kTe have not stayed physical under the capped solve
+    wild = 0
+    for r in runs:
+        ph = r.mesh(cp.MESH_PHI)
+        lo, hi = min(r.p.phi_p, 0.0) - 10.0, max(r.p.phi_p, 0.0) + 10.0
+        wild += int(ph.min() < lo or ph.max() > hi)
     # the runs continue on the plain path, one after another, from the time
     # the batch reached (t accumulated as t += dt, like the device clock)
     t_next = []
@@ -289,9 +300,13 @@ def batched_leg(cp, n_runs, steps, seq_steps, seed, fixed_tile=0):
             "sequential_plain_run_steps_per_s": rs_s,
             "sequential_plain_steps_per_run": seq_steps,
             "speedup_vs_sequential_plain": rs_b / rs_s if rs_s else None,
+            "runs_with_phi_beyond_10kTe_of_range": wild,
             "workload": "256 runs, ncx~U{64..512}, ncy~U{32..256}, ppc~U{20..100}, "
-                        "phi_p~U[-10,2], mi/me 100, Te/Ti 1, Jacobi FP64 residual 1e-6 "
-                        "(max 500 sweeps), one HIP graph per batched step"}
+                        "phi_p~U[-10,2], mi/me 100, Te/Ti 1, Poisson to the FP64 residual "
+                        + ("1e-6 solved directly" if solver == "direct" else
+                           "1e-6 by Jacobi (max 500 sweeps)")
+                        + f"; batch timed over {steps} steps, the plain path over "
+                          f"{seq_steps} steps of every run; one HIP graph per batched step"}
 
 
 def draw_requests(n, rate, seed):
@@ -314,9 +329,13 @@ def converged_rule(prev_net, ie, ii, rel):
     return gross > 0.0 and abs((ii - ie) - prev_net) / gross < rel
 
 
-def serving_leg(cp, n_req, rate, max_steps, seed):
+def serving_leg(cp, n_req, rate, max_steps, seed, plain="full"):
     """Continuous batching (cupic_server) against the same points served one
-    at a time through the plain path with the same convergence rule."""
+    at a time through the plain path with the same convergence rule.  plain
+    "full" steps every point through the plain path and checks that the
+    results are the same; "estimate" times each point's init and 100 plain
+    steps and prices the steps the point needed (the alone-and-in-a-mix
+    identity is a GPU test)."""
     import numpy as np
     reqs = draw_requests(n_req, rate, seed)
     window, rnd, rel = 500, 50, 0.005
@@ -327,6 +346,7 @@ def serving_leg(cp, n_req, rate, max_steps, seed):
     pending = list(range(n_req))
     ids = {}
     arrival = {}
+    resident = []          # (clock, points resident) after every poll
     while True:
         now = srv.stats()["clock"] - start
         while pending and reqs[pending[0]][0] <= now:
@@ -334,6 +354,7 @@ def serving_leg(cp, n_req, rate, max_steps, seed):
             ids[k] = srv.submit(cp.params(**reqs[k][1]))
             arrival[k] = reqs[k][0]
         active, queued = srv.poll()
+        resident.append((srv.stats()["clock"] - start, active))
         if not pending and active == 0 and queued == 0:
             break
         if active == 0 and queued == 0 and pending:
@@ -342,67 +363,98 @@ def serving_leg(cp, n_req, rate, max_steps, seed):
     wall = max(r["t_done"] for r in res.values()) - start
     st = srv.stats()
     srv.close()
-    lat = np.array([res[k]["t_done"] - start - arrival[k] for k in res])
-    conv = sum(r["status"] == cp.POINT_CONVERGED for r in res.values())
-    served = {"points": n_req, "converged": int(conv), "capped": int(n_req - conv),
-              "points_per_s": n_req / wall, "wall_s": wall,
-              "latency_p50_s": float(np.percentile(lat, 50)),
-              "latency_p99_s": float(np.percentile(lat, 99)),
+    lat = {k: res[k]["t_done"] - start - arrival[k] for k in res}
+    conv = [k for k in res if res[k]["status"] == cp.POINT_CONVERGED]
+    # particles moved, approximated by each point's steps x its particles at the end
+    psteps = sum(r["steps"] * sum(r["particles"]) for r in res.values())
+    occ = np.array(resident)
+    mean_res = float(np.sum(occ[1:, 1] * np.diff(occ[:, 0])) / max(occ[-1, 0] - occ[0, 0], 1e-9)) \
+        if len(occ) > 1 else 0.0
+
+    def pct(v, q):
+        return float(np.percentile(v, q)) if len(v) else None
+    served = {"points": n_req, "converged": len(conv), "capped": n_req - len(conv),
+              "points_per_s": n_req / wall,
+              "converged_points_per_s": len(conv) / wall, "wall_s": wall,
+              "latency_converged_p50_s": pct([lat[k] for k in conv], 50),
+              "latency_converged_p99_s": pct([lat[k] for k in conv], 99),
+              "latency_all_p50_s": pct(list(lat.values()), 50),
+              "latency_all_p99_s": pct(list(lat.values()), 99),
+              "steps_converged": sorted(int(res[k]["steps"]) for k in conv),
               "mean_steps": float(np.mean([r["steps"] for r in res.values()])),
+              "particle_steps_per_s": psteps / wall,
+              "mean_resident_points": mean_res,
               "rounds": st["rounds"], "batch_rebuilds": st["rebuilds"]}
     print(f"# served: {served}", file=sys.stderr, flush=True)
-    # plain path: one point at a time, in arrival order, same rule
+    c = srv_cfg
+
+    def plain_params(d):
+        return cp.params(**d, jacobi_mode=c.jacobi_mode, jacobi_tol=c.jacobi_tol,
+                         jacobi_tol_init=c.jacobi_tol, jacobi_max_sweeps=c.jacobi_max_sweeps,
+                         jacobi_check_every=c.jacobi_check_every)
     t_clock = 0.0
     lat_p = []
     same = True
-    t0 = time.perf_counter()
+    busy = 0.0
     for k in range(n_req):
         arr, d = reqs[k]
         t_clock = max(t_clock, arr)
-        c = srv_cfg
-        p = cp.params(**d, jacobi_mode=c.jacobi_mode, jacobi_tol=c.jacobi_tol,
-                      jacobi_tol_init=c.jacobi_tol, jacobi_max_sweeps=c.jacobi_max_sweeps,
-                      jacobi_check_every=c.jacobi_check_every)
         s0 = time.perf_counter()
-        r = cp.Run(p)
+        r = cp.Run(plain_params(d))
         r.init_sim()
-        t = p.dt
-        steps, prev, have, ws = 0, 0.0, False, [0, 0]
-        status = cp.POINT_CAPPED
-        while steps < max_steps:
-            r.step(t)
-            t += p.dt
-            steps += 1
-            if steps % window == 0:
-                tot = r.probe_tally()["total_probe"]
-                ie = (tot[0] - ws[0]) / (window * p.dt)
-                ii = (tot[1] - ws[1]) / (window * p.dt)
-                ws = [int(tot[0]), int(tot[1])]
-                if have and converged_rule(prev, ie, ii, rel):
-                    status = cp.POINT_CONVERGED
-                    break
-                prev, have = ii - ie, True
-        r.synchronize()
-        tot = r.probe_tally()["total_probe"]
-        r.close()
-        t_clock += time.perf_counter() - s0
+        t = r.p.dt
+        if plain == "estimate":
+            r.synchronize()
+            t_init = time.perf_counter() - s0
+            n_probe = min(100, int(res[k]["steps"]))
+            s1 = time.perf_counter()
+            for _ in range(n_probe):
+                r.step(t)
+                t += r.p.dt
+            r.synchronize()
+            per = (time.perf_counter() - s1) / max(n_probe, 1)
+            dt_point = t_init + per * res[k]["steps"]
+            r.close()
+        else:
+            steps, prev, have, ws = 0, 0.0, False, [0, 0]
+            status = cp.POINT_CAPPED
+            while steps < max_steps:
+                r.step(t)
+                t += r.p.dt
+                steps += 1
+                if steps % window == 0:
+                    tot = r.probe_tally()["total_probe"]
+                    ie = (tot[0] - ws[0]) / (window * r.p.dt)
+                    ii = (tot[1] - ws[1]) / (window * r.p.dt)
+                    ws = [int(tot[0]), int(tot[1])]
+                    if have and converged_rule(prev, ie, ii, rel):
+                        status = cp.POINT_CONVERGED
+                        break
+                    prev, have = ii - ie, True
+            r.synchronize()
+            tot = r.probe_tally()["total_probe"]
+            r.close()
+            dt_point = time.perf_counter() - s0
+            same &= (steps == res[k]["steps"] and status == res[k]["status"]
+                     and [int(v) for v in tot] == res[k]["total_probe"])
+        busy += dt_point
+        t_clock += dt_point
         lat_p.append(t_clock - arr)
-        same &= (steps == res[k]["steps"] and status == res[k]["status"]
-                 and [int(v) for v in tot] == res[k]["total_probe"])
-    lat_p = np.array(lat_p)
-    plain = {"points_per_s": n_req / max(t_clock, 1e-9),
-             "latency_p50_s": float(np.percentile(lat_p, 50)),
-             "latency_p99_s": float(np.percentile(lat_p, 99)),
-             "busy_s": time.perf_counter() - t0}
-    return {"workload": f"{n_req} I-V points, Poisson arrivals {rate:g}/s, ncx~U{{128..1024}}, "
-                        f"ncy~U{{64..512}}, ppc~U{{20..200}}, phi_p~U[-10,2]; converged when "
-                        f"the net probe current changes < {rel:g} of the gross current "
-                        f"between {window}-step windows (cap {max_steps} steps); direct FP64 "
-                        f"Poisson solve",
-            "served": served, "plain_one_at_a_time": plain,
-            "points_per_s_vs_plain": served["points_per_s"] / plain["points_per_s"],
-            "results_identical_to_plain": bool(same)}
-
+    plain_res = {"mode": plain, "points_per_s": n_req / max(t_clock, 1e-9),
+                 "converged_points_per_s": len(conv) / max(t_clock, 1e-9),
+                 "latency_converged_p50_s": pct([lat_p[k] for k in conv], 50),
+                 "latency_converged_p99_s": pct([lat_p[k] for k in conv], 99),
+                 "busy_s": busy, "particle_steps_per_s": psteps / max(busy, 1e-9)}
+    out = {"workload": f"{n_req} I-V points, Poisson arrivals {rate:g}/s, ncx~U{{128..1024}}, "
+                       f"ncy~U{{64..512}}, ppc~U{{20..200}}, phi_p~U[-10,2], mi/me 100, "
+                       f"Te/Ti 1; converged when the net probe current changes < {rel:g} of "
+                       f"the gross current between {window}-step windows (cap {max_steps} "
+                       f"steps); direct FP64 Poisson solve",
+           "served": served, "plain_one_at_a_time": plain_res,
+           "points_per_s_vs_plain": served["points_per_s"] / plain_res["points_per_s"]}
+    if plain == "full":
+        out["results_identical_to_plain"] = bool(same)
+    return out
 
 def main():
     ap = argparse.ArgumentParser()
@@ -425,14 +477,21 @@ def main():
     ap.add_argument("--small-steps", type=int, default=1000,
                     help="steps of the small-configuration leg, GPU and CPU (0: skip)")
     ap.add_argument("--batched-runs", type=int, default=256)
-    ap.add_argument("--batched-steps", type=int, default=10)
+    ap.add_argument("--batched-steps", type=int, default=500,
+                    help="timed batched steps (default: one 500-step convergence window)")
     ap.add_argument("--batched-only", action="store_true",
                     help="run only the batched leg (tile from --tile, else tuned; "
                          "sequential baseline from --batched-seq-steps) and print its JSON")
-    ap.add_argument("--batched-seq-steps", type=int, default=3)
-    ap.add_argument("--serve-requests", type=int, default=12)
+    ap.add_argument("--batched-seq-steps", type=int, default=50,
+                    help="timed plain steps of every run for the sequential baseline")
+    ap.add_argument("--serve-requests", type=int, default=50)
     ap.add_argument("--serve-rate", type=float, default=50.0)
-    ap.add_argument("--serve-max-steps", type=int, default=2000)
+    ap.add_argument("--serve-max-steps", type=int, default=10000)
+    ap.add_argument("--serve-plain", default="estimate", choices=["full", "estimate"],
+                    help="plain baseline: every point stepped through the plain path, or "
+                         "each point's init and 100 steps timed and its steps priced")
+    ap.add_argument("--serving-only", action="store_true",
+                    help="run only the serving leg and print its JSON")
     ap.add_argument("--copy-probe-mib", type=int, default=1024,
                     help="size of the plain copy/read/write kernels run first: the achievable "
                          "HBM bandwidth, and the dispatch that scales profiler byte counters "
@@ -454,6 +513,11 @@ def main():
             hbm_probe[kind + "_GBps"] = rate / 1e9
         print(f"# plain streaming kernels over {a.copy_probe_mib} MiB: {hbm_probe}",
               file=sys.stderr, flush=True)
+    if a.serving_only:
+        res = serving_leg(cp, a.serve_requests, a.serve_rate, a.serve_max_steps, 20261203,
+                          a.serve_plain)
+        print(json.dumps({"serving": res}), flush=True)
+        return
     if a.batched_only:
         res = batched_leg(cp, a.batched_runs, a.batched_steps, a.batched_seq_steps,
                           20261203, fixed_tile=a.tile)
@@ -608,7 +672,7 @@ def main():
         print(f"# batched: {out['batched']}", file=sys.stderr, flush=True)
     if a.serve_requests > 0:
         out["serving"] = serving_leg(cp, a.serve_requests, a.serve_rate, a.serve_max_steps,
-                                     20261203)
+                                     20261203, a.serve_plain)
     if not a.no_cpu and a.cpu_max_steps > 0:
         print("# cpu baseline ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(cfg, a.cpu_seconds, a.cpu_max_steps)
