@@ -472,8 +472,9 @@ __device__ __forceinline__ void count_task(const RunDev &R, RunState *st, const 
     em = wave_max_f(em);
     const int c_need =
         hint_code_needed(qdtm, G.dt_f, G.inv_ds, G.ncx, G.ncy, em, __ballot(!fin) == 0ull);
-    // a hint byte at or above this value is trusted (code in the high nibble)
-    const unsigned h_min = mode == kHintOn ? (unsigned)c_need << 4 : 0x100u;
+    // a hint byte is trusted when its code (high nibble) is at least c_need:
+    // code + code_add carries into bit 4 of the byte (c_need 16: never)
+    const unsigned code_add = mode == kHintOn ? (unsigned)(16 - c_need) * 0x01010101u : 0u;
     for (int q = lane; q < nc * 9; q += kWave) cntw[q] = 0;
     for (int q = lane; q < nc; q += kWave) gonew[q] = 0;
     for (int q = lane; q <= nc; q += kWave) bnd[q] = off[c0 + q];
@@ -502,7 +503,7 @@ __device__ __forceinline__ void count_task(const RunDev &R, RunState *st, const 
             }
             const int cl = o.oic - col0;
             if (dir != 4 && cl >= 0 && cl < nc) {
-                if (dir < 9) atomicAdd(&cntw[cl * 9 + dir], 1);
+                if (dir < 9) atomicAdd(&cntw[__mul24(cl, 9) + dir], 1);
                 else atomicAdd(&gonew[cl], 1);
             }
             if (mode == kHintChecked && (int)(hh >> 4) >= c_need && dir != (int)(hh & 15u))
@@ -573,19 +574,35 @@ __device__ __forceinline__ void count_task(const RunDev &R, RunState *st, const 
 #pragma unroll
             for (int q = 0; q < 4; q++) cl[q] += (w0 + 4 * lane + q >= bk);
         }
+        // The four hint bytes are classified together, one flag per byte
+        // (bit 0): valid (slot in the strip), use (code >= c_need, as
+        // code + 16 - c_need >= 16), mov (trusted and direction != 4).
+        unsigned valid = 0x01010101u;
+        if (w0 < p0u || w0 + 4 * kWave > p1u) {  // the strip's first or last window
+            valid = 0u;
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const int i = w0 + 4 * lane + q;
-            const bool valid = i >= p0u && i < p1u;
-            const unsigned h = (word >> (8 * q)) & 0xFFu;
-            const bool use = valid && h >= h_min;
-            const int hdir = (int)(h & 15u);
-            if (use && hdir != 4) atomicAdd(&cntw[cl[q] * 9 + hdir], 1);
-            const bool full = valid && !use;
-            const unsigned long long fq = __ballot(full);
-            if (full) queue[(qh + qn + mbcnt64(fq)) & (kQueue - 1)] = i;
-            qn += __popcll(fq);
-            full_n += __popcll(fq);
+            for (int q = 0; q < 4; q++) {
+                const int i = w0 + 4 * lane + q;
+                valid |= (i >= p0u && i < p1u ? 1u : 0u) << (8 * q);
+            }
+        }
+        const unsigned use = (((word >> 4) & 0x0F0F0F0Fu) + code_add) >> 4 & valid;
+        const unsigned mov = (((word & 0x0F0F0F0Fu) ^ 0x04040404u) + 0x0F0F0F0Fu) >> 4 & use;
+        const unsigned full = valid & ~use;
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+            if ((mov >> (8 * q)) & 1u)
+                atomicAdd(&cntw[__mul24(cl[q], 9) + (int)((word >> (8 * q)) & 15u)], 1);
+        if (__ballot(full != 0u)) {
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const bool f = ((full >> (8 * q)) & 1u) != 0u;
+                const unsigned long long fq = __ballot(f);
+                if (fq == 0ull) continue;
+                if (f) queue[(qh + qn + mbcnt64(fq)) & (kQueue - 1)] = w0 + 4 * lane + q;
+                qn += __popcll(fq);
+                full_n += __popcll(fq);
+            }
         }
         // at most kWave - 1 + 4 kWave entries: the ring never overfills
         while (qn >= kWave) {

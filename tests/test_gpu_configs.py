@@ -10,8 +10,8 @@ shapes, on the GPU through the C-ABI.
   FP64 step from the GPU's state, and over the run the probe current
   against an independent FP64 trajectory, are printed and bounded.
 * Batched configuration: the bench's 256 runs (seed 20261203, grids
-  64x32..512x256, 20..100 ppc, bias -10..+2) in one HIP graph per step;
-  every run's invariants, and 8 runs bit for bit against the same runs
+  64x32..512x256, 20..100 ppc, bias -10..+2, direct FP64 Poisson solve) in
+  one HIP graph per step; every run's invariants, and 8 runs bit for bit against the same runs
   stepped alone.
 * Serving configuration: the first requests of the bench's stream (grids
   128x64..1024x512, 20..200 ppc): each point's result is the same alone and
@@ -173,7 +173,8 @@ def draw_batched(n_runs, seed):
 
 
 def batched_params(d):
-    return cp.params(**d, jacobi_mode=cp.JACOBI_RESIDUAL, jacobi_tol=1e-6,
+    """The batched benchmark's solver: the direct FP64 solve to residual 1e-6."""
+    return cp.params(**d, jacobi_mode=cp.POISSON_DIRECT, jacobi_tol=1e-6,
                      jacobi_tol_init=1e-6, jacobi_max_sweeps=500, jacobi_check_every=50)
 
 
