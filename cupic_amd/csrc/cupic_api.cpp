@@ -110,6 +110,7 @@ int shape_blocks(const RunDev &h, Shape s)
 {
     switch (s) {
     case kShapeNodes: return (h.nnodes + kBlock - 1) / kBlock;
+    case kShapeResid: return (h.nnodes + kBlock * kResNodes - 1) / (kBlock * kResNodes);
     case kShapeTasks: return (2 * h.ncy * h.tpr + 3) / 4;
     case kShapeCountTasks: return (2 * h.ncy * h.tpr1 + 3) / 4;
     case kShapeCellBlk: return 2 * ((h.ncells + kScanCells - 1) / kScanCells);
@@ -126,8 +127,10 @@ int shape_blocks(const RunDev &h, Shape s)
         return ((h.ncx + jac_tx(kJacHNarrow) - 1) / jac_tx(kJacHNarrow)) *
                ((h.ncy - 1 + jac_ty(kJacHNarrow) - 1) / jac_ty(kJacHNarrow));
     case kShapeDftInit: {
-        if (!h.dft_f) return 0;
+        if (!h.dft_d) return 0;
         const int64_t m = h.ncy - 1;
+        if (h.dft_fft)  // twiddles of both lengths, then the eigenvalue parts
+            return (int)((2 * (int64_t)h.ncx + 2 * (int64_t)h.ncy + m + kBlock - 1) / kBlock);
         return (int)(((int64_t)h.ncx * h.ncx + kBlock - 1) / kBlock + (m * m + kBlock - 1) / kBlock +
                      (h.ncx + m + kBlock - 1) / kBlock);
     }
@@ -140,6 +143,10 @@ int shape_blocks(const RunDev &h, Shape s)
     case kShapeDftReduce:
         if (!h.dft_f) return 0;
         return (int)(((int64_t)(h.ncy - 1) * h.ncx + kBlock - 1) / kBlock);
+    case kShapeFftRowsS: return h.dft_fft == 1 ? h.ncy / 2 : 0;  // ceil(m / 2), m = ncy - 1
+    case kShapeFftColsS: return h.dft_fft == 1 ? h.ncx / 2 : 0;
+    case kShapeFftRowsL: return h.dft_fft == 2 ? h.ncy / 2 : 0;
+    case kShapeFftColsL: return h.dft_fft == 2 ? h.ncx / 2 : 0;
     default: return 0;
     }
 }
@@ -228,12 +235,19 @@ void seq_solve(const Target &T, int use_args, int mode, double tol, int min_swee
         return;
     }
     if (mode == 3) {
-        // direct FP64 solve of the same system, then its FP64 residual
+        // direct FP64 solve of the same system, then its FP64 residual: by
+        // FFT where the lengths allow it, else by dense products (in a
+        // batch each run takes its own; launches with no blocks are skipped)
+        for (int step = 0; step < 3; step++) {
+            const bool cols = step == 1;
+            launch_fft(T.L(cols ? kShapeFftColsS : kShapeFftRowsS), step, false);
+            launch_fft(T.L(cols ? kShapeFftColsL : kShapeFftRowsL), step, true);
+        }
         for (int step = 0; step < 4; step++) {
             launch_dft_gemm(T.L(step == 0 || step == 3 ? kShapeDftGemmX : kShapeDftGemmY), step);
             launch_dft_reduce(T.L(kShapeDftReduce), step);
         }
-        launch_residual(N, 0, 2, 0);
+        launch_residual(T.L(kShapeResid), 0, 3, 0);
         launch_solver_end(T.L(kShapeRun), 0, 0);
         return;
     }
@@ -258,7 +272,7 @@ void seq_solve(const Target &T, int use_args, int mode, double tol, int min_swee
         const bool through = !narrow;
         const int step = std::min(per, check);  // so at most one check per launch
         const bool fuse0 = through && check > per && max_sweeps > per;
-        if (!fuse0) launch_residual(N, 0, mode, 0);
+        if (!fuse0) launch_residual(T.L(kShapeResid), 0, mode, 0);
         int cslot = fuse0 ? -1 : 0, cat = 0, cpar = 0;  // the check the next launch reads
         while (s < max_sweeps) {
             int nsw = std::min(step, max_sweeps - s);
@@ -291,7 +305,7 @@ void seq_solve(const Target &T, int use_args, int mode, double tol, int min_swee
     }
     for (;;) {
         if (c > 0) launch_ir_update(N, dpar);
-        launch_residual(N, c, mode, 0);
+        launch_residual(T.L(kShapeResid), c, mode, 0);
         if (s >= max_sweeps) break;
         int k = std::min(check, max_sweeps - s);
         int dl = 0;
@@ -686,9 +700,13 @@ int cupic_run_footprint(const cupic_params *pin, int64_t capacity_hint, int64_t 
     b += (int64_t)(p.jacobi_max_sweeps + 2) * 12 * kRedSlots;
     if (p.jacobi_mode == 3) {
         const int64_t nx = p.ncx, m = p.ncy - 1;
-        const int64_t ns = std::max(dft_slices(p.ncx, p.ncy - 1, p.ncx),
-                                    dft_slices(p.ncx, p.ncy - 1, p.ncy - 1));
-        b += 8 * (2 * nx * nx + m * m + nx + m + (2 + ns) * m * nx);
+        if (fft_class(p.ncx, p.ncy)) {
+            b += 8 * (2 * (nx + 2 * (int64_t)p.ncy) + nx + m + 2 * m * nx);
+        } else {
+            const int64_t ns = std::max(dft_slices(p.ncx, p.ncy - 1, p.ncx),
+                                        dft_slices(p.ncx, p.ncy - 1, p.ncy - 1));
+            b += 8 * (2 * nx * nx + m * m + nx + m + (2 + ns) * m * nx);
+        }
     }
     *bytes = b;
     return CUPIC_OK;
@@ -787,15 +805,20 @@ int cupic_run_create(const cupic_params *pin, int64_t capacity_hint, cupic_run *
     h.resid = dev_alloc<unsigned long long>(r, ((size_t)p.jacobi_max_sweeps + 2) * kRedSlots, &st);
     if (p.jacobi_mode == 3) {
         const size_t nx = (size_t)p.ncx, m = (size_t)p.ncy - 1;
-        h.nsx = dft_slices(p.ncx, p.ncy - 1, p.ncx);
-        h.nsy = dft_slices(p.ncx, p.ncy - 1, p.ncy - 1);
-        h.dft_f = dev_alloc<double>(r, nx * nx, &st);
-        h.dft_g = dev_alloc<double>(r, nx * nx, &st);
-        h.dft_s = dev_alloc<double>(r, m * m, &st);
+        h.dft_fft = fft_class(p.ncx, p.ncy);
+        if (h.dft_fft) {
+            h.dft_tw = dev_alloc<double>(r, 2 * (nx + 2 * (size_t)p.ncy), &st);
+        } else {
+            h.nsx = dft_slices(p.ncx, p.ncy - 1, p.ncx);
+            h.nsy = dft_slices(p.ncx, p.ncy - 1, p.ncy - 1);
+            h.dft_f = dev_alloc<double>(r, nx * nx, &st);
+            h.dft_g = dev_alloc<double>(r, nx * nx, &st);
+            h.dft_s = dev_alloc<double>(r, m * m, &st);
+            h.dft_p = dev_alloc<double>(r, (size_t)std::max(h.nsx, h.nsy) * m * nx, &st);
+        }
         h.dft_d = dev_alloc<double>(r, nx + m, &st);
         h.dft_z = dev_alloc<double>(r, m * nx, &st);
         h.dft_y = dev_alloc<double>(r, m * nx, &st);
-        h.dft_p = dev_alloc<double>(r, (size_t)std::max(h.nsx, h.nsy) * m * nx, &st);
     }
     h.st = dev_alloc<RunState>(r, 1, &st);
     r->d_rd = dev_alloc<RunDev>(r, 1, &st);
@@ -821,7 +844,7 @@ int cupic_run_create(const cupic_params *pin, int64_t capacity_hint, cupic_run *
         cupic_run_destroy(r);
         return set_err(nullptr, CUPIC_ERR_HIP, "run setup failed");
     }
-    if (h.dft_f) {
+    if (h.dft_d) {
         launch_dft_init(target_of(r).L(kShapeDftInit));
         if (hipStreamSynchronize(r->stream) != hipSuccess) {
             cupic_run_destroy(r);

@@ -38,12 +38,26 @@ constexpr int kTileW = kCmax + 3;   // deposit tile width (node columns)
 constexpr int kCountCells = 32;     // cells per wave strip of the counting pass (any tile)
 constexpr int kScanCells = 1024;    // cells per scan block
 constexpr int kRedSlots = 16;       // atomic slots per reduced quantity
+constexpr int kResNodes = 8;        // FP64 residual: nodes per thread
 constexpr int kFarBlocks = 16;      // far-list kernels: at most this many blocks per species
 constexpr int kGM = 64;             // direct solve: GEMM output tile rows (4 waves x 16)
 constexpr int kGN = 64;             // direct solve: GEMM output tile columns (4 chains per wave)
 constexpr int kGK = 32;             // direct solve: GEMM k tile
 constexpr int kGSlices = 512;       // direct solve: workgroups a product aims for (k slices)
 constexpr int kGMaxSlices = 8;
+// Direct solve by FFT: transform lengths (ncx along x, 2 ncy along y) that
+// are powers of two up to kFftL; two kernel instances, for lengths up to
+// kFftS and up to kFftL (LDS: two arrays of the length in complex doubles).
+constexpr int kFftS = 1024;
+constexpr int kFftL = 4096;
+__host__ __device__ constexpr bool fft_len_ok(int n) { return n >= 2 && n <= kFftL && (n & (n - 1)) == 0; }
+// 0: the dense products; 1: FFT, lengths up to kFftS; 2: FFT, up to kFftL
+__host__ __device__ constexpr int fft_class(int ncx, int ncy)
+{
+    return !(ncy >= 2 && fft_len_ok(ncx) && fft_len_ok(2 * ncy))
+               ? 0
+               : (ncx <= kFftS && 2 * ncy <= kFftS ? 1 : 2);
+}
 constexpr int kJacRows = 8;         // blocked Jacobi: rows per wave
 constexpr int kJacWaves = 8;        // blocked Jacobi: waves per workgroup
 constexpr int kJacThreads = kJacWaves * 64;
@@ -177,6 +191,9 @@ struct RunDev {
     double *dft_d;                  // mode 3: [ncx + m] eigenvalue parts per column, per row
     double *dft_z, *dft_y;          // mode 3: [m][ncx] transformed rows (two stages)
     double *dft_p;                  // mode 3: [max(nsx, nsy)][m][ncx] partial sums
+    int dft_fft;                    // mode 3: fft_class(ncx, ncy) (0: the dense products)
+    double *dft_tw;                 // mode 3, FFT: twiddles e^(-2 pi i k / L), L = ncx then
+                                    // 2 ncy (re, im pairs)
     RunState *st;
 };
 

@@ -22,6 +22,7 @@ struct Launch {
 // Work shapes: blocks per run for each kernel family.
 enum Shape : int {
     kShapeNodes = 0,   // ceil(nnodes / 256)
+    kShapeResid,       // ceil(nnodes / (256 kResNodes))
     kShapeTasks,       // ceil(2 * ncy * tpr / 4): one wave per (species, row, column block)
     kShapeCountTasks,  // ceil(2 * ncy * tpr1 / 4): the same for the counting pass
     kShapeCellBlk,     // 2 * ceil(ncells / kScanCells)
@@ -37,6 +38,10 @@ enum Shape : int {
     kShapeDftGemmX,    // ceil(m / kGM) * ceil(ncx / kGN) * nsx
     kShapeDftGemmY,    // ceil(m / kGM) * ceil(ncx / kGN) * nsy
     kShapeDftReduce,   // ceil(m * ncx / 256)
+    kShapeFftRowsS,    // ceil(m / 2) when fft_class is 1: row pairs
+    kShapeFftColsS,    // ncx / 2 when fft_class is 1: column pairs
+    kShapeFftRowsL,    // the same for fft_class 2
+    kShapeFftColsL,
     kShapeCount
 };
 
@@ -59,6 +64,9 @@ void launch_field(const Launch &L);
 void launch_dft_init(const Launch &L);
 void launch_dft_gemm(const Launch &L, int step);
 void launch_dft_reduce(const Launch &L, int step);
+// step 0: forward x transform of row pairs; 1: the y solve of column pairs;
+// 2: inverse x transform into the potential.  large: the kFftL instance.
+void launch_fft(const Launch &L, int step, bool large);
 
 // particles
 void launch_create(const Launch &L);

@@ -140,6 +140,29 @@ __global__ __launch_bounds__(kBlock) void k_dft_init(const RunDev *rds, const in
 {
     RUN_PROLOGUE();
     const int ncx = R.ncx, m = R.ncy - 1;
+    if (R.dft_fft) {
+        // twiddles e^(-2 pi i t / L) for L = ncx, then L = 2 ncy (t / L exact),
+        // then the eigenvalue parts as below
+        const int L2 = 2 * R.ncy;
+        int e = local * kBlock + threadIdx.x;
+        double2 *tw = reinterpret_cast<double2 *>(R.dft_tw);
+        if (e < ncx + L2) {
+            const int L = e < ncx ? ncx : L2, t = e < ncx ? e : e - ncx;
+            const double a = 2.0 * (double)t / (double)L;
+            tw[e] = make_double2(cospi(a), -sinpi(a));
+            return;
+        }
+        e -= ncx + L2;
+        if (e < ncx) {
+            const double sk = sinpi((double)dft_k_of(e, ncx) / (double)ncx);
+            R.dft_d[e] = 4.0 * sk * sk;
+        } else if (e < ncx + m) {
+            const int l = e - ncx;
+            const double sl = sinpi((double)(l + 1) / (2.0 * (double)(m + 1)));
+            R.dft_d[e] = 4.0 * sl * sl;
+        }
+        return;
+    }
     const long long nfg = (long long)ncx * ncx, nss = (long long)m * m;
     const int bfg = (int)((nfg + kBlock - 1) / kBlock), bss = (int)((nss + kBlock - 1) / kBlock);
     const long long e0 = (long long)local * kBlock + threadIdx.x;
@@ -247,6 +270,259 @@ __global__ __launch_bounds__(kBlock) void k_dft_reduce(const RunDev *rds, const 
     }
 }
 
+// ---------------------------------------------------------------- FFT path
+//
+// With ncx and 2 ncy powers of two (up to kFftL) the same solve runs as fast
+// transforms, three kernels of one workgroup per row or column pair:
+//   0. rows r, r+1 of R (the right-hand side with the Dirichlet rows added,
+//      as for the dense product) as one complex row z = R_r + i R_{r+1}; a
+//      forward FFT of length ncx split into the two real spectra
+//      X1 = (Z_k + conj Z_{-k}) / 2, X2 = (Z_k - conj Z_{-k}) / 2i, stored as
+//      the dense path's ncx real coefficients per row, column-major
+//      (dft_z[c][r]): Re X_k in the cos columns, Im X_k in the sin columns;
+//   1. columns c, c+1 as one complex column x + i y: the sine transform of
+//      length m as an FFT of length 2(m+1) of the odd extension
+//      (0, v_0 .. v_{m-1}, 0, -v_{m-1} .. -v_0), whose coefficient l+1 is
+//      -2i X_l + 2 Y_l; scaled by sqrt(2/(m+1)) and divided by mu(l, k),
+//      then transformed again the same way (S is its own inverse), into
+//      dft_y[c][r];
+//   2. rows r, r+1: both real spectra as one complex spectrum extended by
+//      Hermitian symmetry, transformed back (conjugate, forward FFT,
+//      conjugate, / ncx) into the two rows of the potential.
+// The FFT: iterative Stockham in LDS (natural order in and out), radix-4
+// stages and, when log2 L is odd, a last radix-2 stage; each stage reads
+// its butterflies into registers, synchronises and writes in place.  The
+// twiddles e^(-2 pi i t / L) come from the run's table (set up once).
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b)
+{
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+template <int LMAX>
+__device__ void fft_lds(double2 *buf, const double2 *tw, int L)
+{
+    constexpr int kB4 = (LMAX / 4 + kBlock - 1) / kBlock;  // radix-4 butterflies per thread
+    constexpr int kB2 = (LMAX / 2 + kBlock - 1) / kBlock;
+    const int lg = 31 - __clz(L);
+    int Ns = 1;
+    for (int stage = 0; stage < lg / 2; stage++) {
+        const int nb = L >> 2, ts = L / (4 * Ns);
+        double2 o[kB4][4];
+#pragma unroll
+        for (int u = 0; u < kB4; u++) {
+            const int j = threadIdx.x + u * kBlock;
+            if (j < nb) {
+                const int k = j & (Ns - 1);
+                const double2 v0 = buf[j];
+                const double2 v1 = cmul(buf[j + nb], tw[k * ts]);
+                const double2 v2 = cmul(buf[j + 2 * nb], tw[2 * k * ts]);
+                const double2 v3 = cmul(buf[j + 3 * nb], tw[3 * k * ts]);
+                const double2 a = make_double2(v0.x + v2.x, v0.y + v2.y);
+                const double2 b = make_double2(v0.x - v2.x, v0.y - v2.y);
+                const double2 c = make_double2(v1.x + v3.x, v1.y + v3.y);
+                const double2 d = make_double2(v1.y - v3.y, v3.x - v1.x);  // -i (v1 - v3)
+                o[u][0] = make_double2(a.x + c.x, a.y + c.y);
+                o[u][1] = make_double2(b.x + d.x, b.y + d.y);
+                o[u][2] = make_double2(a.x - c.x, a.y - c.y);
+                o[u][3] = make_double2(b.x - d.x, b.y - d.y);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < kB4; u++) {
+            const int j = threadIdx.x + u * kBlock;
+            if (j < nb) {
+                const int k = j & (Ns - 1);
+                const int d = (j - k) * 4 + k;
+#pragma unroll
+                for (int r = 0; r < 4; r++) buf[d + r * Ns] = o[u][r];
+            }
+        }
+        __syncthreads();
+        Ns *= 4;
+    }
+    if (lg & 1) {
+        const int nb = L >> 1;  // Ns = L / 2: twiddle stride 1
+        double2 o[kB2][2];
+#pragma unroll
+        for (int u = 0; u < kB2; u++) {
+            const int j = threadIdx.x + u * kBlock;
+            if (j < nb) {
+                const int k = j & (Ns - 1);
+                const double2 v0 = buf[j];
+                const double2 v1 = cmul(buf[j + nb], tw[k]);
+                o[u][0] = make_double2(v0.x + v1.x, v0.y + v1.y);
+                o[u][1] = make_double2(v0.x - v1.x, v0.y - v1.y);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < kB2; u++) {
+            const int j = threadIdx.x + u * kBlock;
+            if (j < nb) {
+                const int k = j & (Ns - 1);
+                const int d = (j - k) * 2 + k;
+                buf[d] = o[u][0];
+                buf[d + Ns] = o[u][1];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// step 0: row pair g -> the two rows' real spectra, column-major in dft_z
+template <int LMAX>
+__device__ void fft_rows_fwd(const RunDev &R, int g, double2 *buf, double2 *tw)
+{
+    const int N = R.ncx, ncy = R.ncy, m = ncy - 1;
+    const int r1 = 2 * g, r2 = r1 + 1;
+    const CUPIC_GLOBAL double *b = gp(R.b64);
+    const CUPIC_GLOBAL double *phi = gp(R.phi64);
+    const CUPIC_GLOBAL double *twg = gp(R.dft_tw);
+    auto row = [&](int r, int i) -> double {
+        if (r >= m) return 0.0;
+        double v = b[(size_t)(r + 1) * N + i];
+        if (r == 0) v += phi[i];                          // phi row 0 (probe)
+        if (r == m - 1) v += phi[(size_t)ncy * N + i];    // phi row ncy
+        return v;
+    };
+    for (int i = threadIdx.x; i < N; i += kBlock) {
+        tw[i] = make_double2(twg[2 * i], twg[2 * i + 1]);
+        buf[i] = make_double2(row(r1, i), row(r2, i));
+    }
+    __syncthreads();
+    fft_lds<LMAX>(buf, tw, N);
+    const int kc = dft_cos_cols(N), ks = N - kc;
+    CUPIC_GLOBAL double *Z = gp(R.dft_z);
+    for (int k = threadIdx.x; k <= N / 2; k += kBlock) {
+        const double2 a = buf[k], c = buf[(N - k) & (N - 1)];
+        const double x1r = 0.5 * (a.x + c.x), x1i = 0.5 * (a.y - c.y);
+        const double x2r = 0.5 * (a.y + c.y), x2i = -0.5 * (a.x - c.x);
+        Z[(size_t)k * m + r1] = x1r;
+        if (r2 < m) Z[(size_t)k * m + r2] = x2r;
+        if (k >= 1 && k <= ks) {
+            Z[(size_t)(kc + k - 1) * m + r1] = x1i;
+            if (r2 < m) Z[(size_t)(kc + k - 1) * m + r2] = x2i;
+        }
+    }
+}
+
+// step 1: column pair g: Y = S R^ / mu, then X^ = S Y, column-major in dft_y
+template <int LMAX>
+__device__ void fft_cols(const RunDev &R, int g, double2 *buf, double2 *tw)
+{
+    constexpr int kU = (LMAX / 2 + kBlock - 1) / kBlock;  // m < LMAX / 2
+    const int N = R.ncx, M = R.ncy, m = M - 1, L = 2 * M;
+    const int c1 = 2 * g, c2 = c1 + 1;
+    const CUPIC_GLOBAL double *twg = gp(R.dft_tw) + 2 * N;
+    const CUPIC_GLOBAL double *Z = gp(R.dft_z);
+    const CUPIC_GLOBAL double *D = gp(R.dft_d);
+    const double sc = sqrt(2.0 / (double)M);
+    // odd extension of column values x_j + i y_j (j < m) into buf
+    auto extend = [&](auto val) {
+        for (int n = threadIdx.x; n < L; n += kBlock) {
+            double2 v = make_double2(0.0, 0.0);
+            if (n > 0 && n < M) {
+                v = val(n - 1);
+            } else if (n > M) {
+                const double2 w = val(L - n - 1);
+                v = make_double2(-w.x, -w.y);
+            }
+            buf[n] = v;
+        }
+    };
+    for (int n = threadIdx.x; n < L; n += kBlock) tw[n] = make_double2(twg[2 * n], twg[2 * n + 1]);
+    extend([&](int j) { return make_double2(Z[(size_t)c1 * m + j], Z[(size_t)c2 * m + j]); });
+    __syncthreads();
+    fft_lds<LMAX>(buf, tw, L);
+    double2 s[kU];
+#pragma unroll
+    for (int u = 0; u < kU; u++) {
+        const int l = threadIdx.x + u * kBlock;
+        if (l < m) {
+            const double2 V = buf[l + 1];
+            const double dl = D[N + l];
+            s[u] = make_double2((sc * (-0.5 * V.y)) / (D[c1] + dl), (sc * (0.5 * V.x)) / (D[c2] + dl));
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) buf[0] = buf[M] = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int u = 0; u < kU; u++) {
+        const int l = threadIdx.x + u * kBlock;
+        if (l < m) {
+            buf[l + 1] = s[u];
+            buf[L - l - 1] = make_double2(-s[u].x, -s[u].y);
+        }
+    }
+    __syncthreads();
+    fft_lds<LMAX>(buf, tw, L);
+    CUPIC_GLOBAL double *Y = gp(R.dft_y);
+    for (int j = threadIdx.x; j < m; j += kBlock) {
+        const double2 V = buf[j + 1];
+        Y[(size_t)c1 * m + j] = sc * (-0.5 * V.y);
+        Y[(size_t)c2 * m + j] = sc * (0.5 * V.x);
+    }
+}
+
+// step 2: row pair g of X^ -> rows r1 + 1, r2 + 1 of the FP64 potential and
+// its FP32 copy
+template <int LMAX>
+__device__ void fft_rows_inv(const RunDev &R, const RunState *st, int g, double2 *buf,
+                             double2 *tw)
+{
+    const int N = R.ncx, m = R.ncy - 1;
+    const int r1 = 2 * g, r2 = r1 + 1;
+    const CUPIC_GLOBAL double *twg = gp(R.dft_tw);
+    const CUPIC_GLOBAL double *X = gp(R.dft_y);
+    const int kc = dft_cos_cols(N), ks = N - kc;
+    // coefficient k of row r as a complex number (Hermitian beyond N / 2)
+    auto coef = [&](int r, int k) -> double2 {
+        if (r >= m) return make_double2(0.0, 0.0);
+        const int kk = k <= N / 2 ? k : N - k;
+        const double c = X[(size_t)kk * m + r];
+        const double s = (kk >= 1 && kk <= ks) ? X[(size_t)(kc + kk - 1) * m + r] : 0.0;
+        return make_double2(c, k <= N / 2 ? s : -s);
+    };
+    for (int k = threadIdx.x; k < N; k += kBlock) {
+        tw[k] = make_double2(twg[2 * k], twg[2 * k + 1]);
+        const double2 a = coef(r1, k), b = coef(r2, k);
+        // conj(a + i b)
+        buf[k] = make_double2(a.x - b.y, -(a.y + b.x));
+    }
+    __syncthreads();
+    fft_lds<LMAX>(buf, tw, N);
+    const double inv = 1.0 / (double)N;  // a power of two: exact
+    CUPIC_GLOBAL double *phi = gp(R.phi64);
+    CUPIC_GLOBAL float *phi32 = gp(R.phi32[st->phi_cur]);
+    for (int i = threadIdx.x; i < N; i += kBlock) {
+        const double2 z = buf[i];
+        const double v1 = z.x * inv, v2 = -z.y * inv;
+        phi[(size_t)(r1 + 1) * N + i] = v1;
+        phi32[(size_t)(r1 + 1) * N + i] = (float)v1;
+        if (r2 < m) {
+            phi[(size_t)(r2 + 1) * N + i] = v2;
+            phi32[(size_t)(r2 + 1) * N + i] = (float)v2;
+        }
+    }
+}
+
+template <int LMAX>
+__global__ __launch_bounds__(kBlock) void k_fft(const RunDev *rds, const int *start, int nruns,
+                                                int step)
+{
+    RUN_PROLOGUE();
+    __shared__ double2 buf[LMAX];
+    __shared__ double2 tw[LMAX];
+    if (step == 0)
+        fft_rows_fwd<LMAX>(R, local, buf, tw);
+    else if (step == 1)
+        fft_cols<LMAX>(R, local, buf, tw);
+    else
+        fft_rows_inv<LMAX>(R, st, local, buf, tw);
+}
+
 #define LAUNCH_K(kern, L, ...)                                                         \
     do {                                                                               \
         if ((L).nblocks > 0)                                                           \
@@ -257,5 +533,12 @@ __global__ __launch_bounds__(kBlock) void k_dft_reduce(const RunDev *rds, const 
 void launch_dft_init(const Launch &L) { LAUNCH_K(k_dft_init, L); }
 void launch_dft_gemm(const Launch &L, int step) { LAUNCH_K(k_dft_gemm, L, step); }
 void launch_dft_reduce(const Launch &L, int step) { LAUNCH_K(k_dft_reduce, L, step); }
+void launch_fft(const Launch &L, int step, bool large)
+{
+    if (large)
+        LAUNCH_K(k_fft<kFftL>, L, step);
+    else
+        LAUNCH_K(k_fft<kFftS>, L, step);
+}
 
 }  // namespace cupic

@@ -405,34 +405,39 @@ __global__ __launch_bounds__(kBlock) void k_ir_update(const RunDev *rds, const i
 // FP64 residual max over interior nodes of |0.25 (b + sum of neighbours) - phi|,
 // the same measure as the reference's max |dphi| (src/mesh.cu:354).  Mode 1
 // evaluates it on the FP32 phi in buffer par; mode 2 on the FP64 master and
-// also stores the correction right-hand side r = b + sum - 4 phi.
+// also stores the correction right-hand side r = b + sum - 4 phi (mode 3
+// reads the FP64 potential too).  Each thread takes kResNodes nodes, a
+// block stride apart, so that few blocks meet at the atomic slots.
 __global__ __launch_bounds__(kBlock) void k_residual(const RunDev *rds, const int *start,
                                                      int nruns, int c, int mode, int par)
 {
     RUN_PROLOGUE();
     __shared__ double sh[kBlock / kWave];
     if (st->solve_done) return;
-    int k = local * kBlock + threadIdx.x;
+    const int ncx = R.ncx;
+    const CUPIC_GLOBAL double *p64 = gp(R.phi64);
+    const CUPIC_GLOBAL float *p32 = gp(R.phi32[phi_at(st->phi_cur, par)]);
+    const CUPIC_GLOBAL double *b64 = gp(R.b64);
     double rmax = 0.0;
-    if (k < R.nnodes) {
-        int ncx = R.ncx;
-        int j = k / ncx;
-        int i = k - j * ncx;
+#pragma unroll
+    for (int u = 0; u < kResNodes; u++) {
+        const int k = (local * kResNodes + u) * kBlock + threadIdx.x;
+        if (k >= R.nnodes) break;
+        const int j = k / ncx;
+        const int i = k - j * ncx;
         if (j >= 1 && j < R.ncy) {
-            int kw = (i == 0) ? k + ncx - 1 : k - 1;
-            int ke = (i == ncx - 1) ? k - ncx + 1 : k + 1;
+            const int kw = (i == 0) ? k + ncx - 1 : k - 1;
+            const int ke = (i == ncx - 1) ? k - ncx + 1 : k + 1;
             double pc_, pw, pe, pn, ps;
-            if (mode == 2) {
-                const CUPIC_GLOBAL double *p = gp(R.phi64);
-                pc_ = p[k]; pw = p[kw]; pe = p[ke]; pn = p[k + ncx]; ps = p[k - ncx];
+            if (mode >= 2) {
+                pc_ = p64[k]; pw = p64[kw]; pe = p64[ke]; pn = p64[k + ncx]; ps = p64[k - ncx];
             } else {
-                const CUPIC_GLOBAL float *p = gp(R.phi32[phi_at(st->phi_cur, par)]);
-                pc_ = p[k]; pw = p[kw]; pe = p[ke]; pn = p[k + ncx]; ps = p[k - ncx];
+                pc_ = p32[k]; pw = p32[kw]; pe = p32[ke]; pn = p32[k + ncx]; ps = p32[k - ncx];
             }
-            double sum = (((gp(R.b64)[k] + pw) + pe) + pn) + ps;
-            double rr = sum - 4.0 * pc_;
+            const double sum = (((b64[k] + pw) + pe) + pn) + ps;
+            const double rr = sum - 4.0 * pc_;
             if (mode == 2) gp(R.r32)[k] = (float)rr;
-            rmax = fabs(0.25 * rr);
+            rmax = fmax(rmax, fabs(0.25 * rr));
         }
     }
     unsigned long long *slot = &R.resid[c * kRedSlots + (local & (kRedSlots - 1))];
