@@ -285,7 +285,8 @@ __device__ __forceinline__ void charge_to_rhs(const RunDev &R, long long a, int 
 // start[0] is 0, so a launch for one run needs no load from start.
 #define RUN_PROLOGUE()                                                  \
     const int b_ = blockIdx.x;                                          \
-    const int r_ = nruns == 1 ? 0 : find_run(start, nruns, b_);         \
+    const int r_ = __builtin_amdgcn_readfirstlane(                      \
+        nruns == 1 ? 0 : find_run(start, nruns, b_));  /* uniform */    \
     const RunDev &R = rds[r_];                                          \
     const int local = nruns == 1 ? b_ : b_ - start[r_];                 \
     RunState *st = R.st;                                                \

@@ -371,25 +371,46 @@ __device__ void fft_lds(double2 *buf, const double2 *tw, int L)
     }
 }
 
+// The global loads of a kernel are all issued before the first LDS store
+// (kE elements per thread in registers), so that they cost one round trip.
+
 // step 0: row pair g -> the two rows' real spectra, column-major in dft_z
 template <int LMAX>
 __device__ void fft_rows_fwd(const RunDev &R, int g, double2 *buf, double2 *tw)
 {
+    constexpr int kE = LMAX / kBlock;
     const int N = R.ncx, ncy = R.ncy, m = ncy - 1;
     const int r1 = 2 * g, r2 = r1 + 1;
     const CUPIC_GLOBAL double *b = gp(R.b64);
     const CUPIC_GLOBAL double *phi = gp(R.phi64);
     const CUPIC_GLOBAL double *twg = gp(R.dft_tw);
-    auto row = [&](int r, int i) -> double {
-        if (r >= m) return 0.0;
-        double v = b[(size_t)(r + 1) * N + i];
-        if (r == 0) v += phi[i];                          // phi row 0 (probe)
-        if (r == m - 1) v += phi[(size_t)ncy * N + i];    // phi row ncy
-        return v;
-    };
-    for (int i = threadIdx.x; i < N; i += kBlock) {
-        tw[i] = make_double2(twg[2 * i], twg[2 * i + 1]);
-        buf[i] = make_double2(row(r1, i), row(r2, i));
+    // rows r1 and r2 (r2 clamped: row m does not exist and reads as 0) and
+    // the Dirichlet rows, loaded without conditions so that no load waits
+    // for another; the Dirichlet values are added to rows 0 and m - 1
+    const int ra = r1 + 1, rb = min(r2, m - 1) + 1;
+    double2 t[kE], v[kE], pd[kE];
+#pragma unroll
+    for (int u = 0; u < kE; u++) {
+        const int i = min((int)threadIdx.x + u * kBlock, N - 1);
+        t[u] = make_double2(twg[2 * i], twg[2 * i + 1]);
+        v[u] = make_double2(b[(size_t)ra * N + i], b[(size_t)rb * N + i]);
+        pd[u] = make_double2(phi[i], phi[(size_t)ncy * N + i]);
+    }
+#pragma unroll
+    for (int u = 0; u < kE; u++) {
+        double x = v[u].x, y = r2 < m ? v[u].y : 0.0;
+        if (r1 == 0) x += pd[u].x;                   // phi row 0 (probe)
+        if (r1 == m - 1) x += pd[u].y;               // phi row ncy
+        if (r2 == m - 1) y += pd[u].y;
+        v[u] = make_double2(x, y);
+    }
+#pragma unroll
+    for (int u = 0; u < kE; u++) {
+        const int i = threadIdx.x + u * kBlock;
+        if (i < N) {
+            tw[i] = t[u];
+            buf[i] = v[u];
+        }
     }
     __syncthreads();
     fft_lds<LMAX>(buf, tw, N);
@@ -412,6 +433,7 @@ __device__ void fft_rows_fwd(const RunDev &R, int g, double2 *buf, double2 *tw)
 template <int LMAX>
 __device__ void fft_cols(const RunDev &R, int g, double2 *buf, double2 *tw)
 {
+    constexpr int kE = LMAX / kBlock;
     constexpr int kU = (LMAX / 2 + kBlock - 1) / kBlock;  // m < LMAX / 2
     const int N = R.ncx, M = R.ncy, m = M - 1, L = 2 * M;
     const int c1 = 2 * g, c2 = c1 + 1;
@@ -419,31 +441,48 @@ __device__ void fft_cols(const RunDev &R, int g, double2 *buf, double2 *tw)
     const CUPIC_GLOBAL double *Z = gp(R.dft_z);
     const CUPIC_GLOBAL double *D = gp(R.dft_d);
     const double sc = sqrt(2.0 / (double)M);
-    // odd extension of column values x_j + i y_j (j < m) into buf
-    auto extend = [&](auto val) {
-        for (int n = threadIdx.x; n < L; n += kBlock) {
-            double2 v = make_double2(0.0, 0.0);
-            if (n > 0 && n < M) {
-                v = val(n - 1);
-            } else if (n > M) {
-                const double2 w = val(L - n - 1);
-                v = make_double2(-w.x, -w.y);
-            }
-            buf[n] = v;
+    // the odd extension (0, v_0 .. v_{m-1}, 0, -v_{m-1} .. -v_0) of the
+    // column values v_j = x_j + i y_j: element n < M is v_{n-1}, element
+    // n > M is -v_{L-n-1}
+    double2 t[kE], v[kE];
+    double dl[kU];
+    const double d1 = D[c1], d2 = D[c2];
+#pragma unroll
+    for (int u = 0; u < kE; u++) {
+        const int n = min((int)threadIdx.x + u * kBlock, L - 1);
+        const int j = min(max(n < M ? n - 1 : L - n - 1, 0), m - 1);  // clamped for n = 0, M
+        t[u] = make_double2(twg[2 * n], twg[2 * n + 1]);
+        v[u] = make_double2(Z[(size_t)c1 * m + j], Z[(size_t)c2 * m + j]);
+    }
+#pragma unroll
+    for (int u = 0; u < kE; u++) {
+        const int n = threadIdx.x + u * kBlock;
+        v[u] = (n == 0 || n == M) ? make_double2(0.0, 0.0)
+                                  : (n < M ? v[u] : make_double2(-v[u].x, -v[u].y));
+    }
+#pragma unroll
+    for (int u = 0; u < kU; u++) {
+        const int l = threadIdx.x + u * kBlock;
+        if (l < m) dl[u] = D[N + l];
+    }
+#pragma unroll
+    for (int u = 0; u < kE; u++) {
+        const int n = threadIdx.x + u * kBlock;
+        if (n < L) {
+            tw[n] = t[u];
+            buf[n] = v[u];
         }
-    };
-    for (int n = threadIdx.x; n < L; n += kBlock) tw[n] = make_double2(twg[2 * n], twg[2 * n + 1]);
-    extend([&](int j) { return make_double2(Z[(size_t)c1 * m + j], Z[(size_t)c2 * m + j]); });
+    }
     __syncthreads();
     fft_lds<LMAX>(buf, tw, L);
+    // coefficient l + 1 is -2i X_l + 2 Y_l: X_l = -Im / 2, Y_l = Re / 2
     double2 s[kU];
 #pragma unroll
     for (int u = 0; u < kU; u++) {
         const int l = threadIdx.x + u * kBlock;
         if (l < m) {
             const double2 V = buf[l + 1];
-            const double dl = D[N + l];
-            s[u] = make_double2((sc * (-0.5 * V.y)) / (D[c1] + dl), (sc * (0.5 * V.x)) / (D[c2] + dl));
+            s[u] = make_double2((sc * (-0.5 * V.y)) / (d1 + dl[u]), (sc * (0.5 * V.x)) / (d2 + dl[u]));
         }
     }
     __syncthreads();
@@ -472,24 +511,46 @@ template <int LMAX>
 __device__ void fft_rows_inv(const RunDev &R, const RunState *st, int g, double2 *buf,
                              double2 *tw)
 {
+    constexpr int kE = LMAX / kBlock;
     const int N = R.ncx, m = R.ncy - 1;
     const int r1 = 2 * g, r2 = r1 + 1;
     const CUPIC_GLOBAL double *twg = gp(R.dft_tw);
     const CUPIC_GLOBAL double *X = gp(R.dft_y);
     const int kc = dft_cos_cols(N), ks = N - kc;
     // coefficient k of row r as a complex number (Hermitian beyond N / 2)
-    auto coef = [&](int r, int k) -> double2 {
-        if (r >= m) return make_double2(0.0, 0.0);
+    // rows r1 and r2 (r2 clamped, then zeroed when it is row m) of the cos
+    // coefficient kk and the sin coefficient kk (index clamped, zeroed
+    // outside 1 .. ks), loaded without conditions
+    const int rb = min(r2, m - 1);
+    double2 t[kE], v[kE];
+    double4 q[kE];
+#pragma unroll
+    for (int u = 0; u < kE; u++) {
+        const int k = min((int)threadIdx.x + u * kBlock, N - 1);
         const int kk = k <= N / 2 ? k : N - k;
-        const double c = X[(size_t)kk * m + r];
-        const double s = (kk >= 1 && kk <= ks) ? X[(size_t)(kc + kk - 1) * m + r] : 0.0;
-        return make_double2(c, k <= N / 2 ? s : -s);
-    };
-    for (int k = threadIdx.x; k < N; k += kBlock) {
-        tw[k] = make_double2(twg[2 * k], twg[2 * k + 1]);
-        const double2 a = coef(r1, k), b = coef(r2, k);
-        // conj(a + i b)
-        buf[k] = make_double2(a.x - b.y, -(a.y + b.x));
+        const int cs = kc + min(max(kk, 1), max(ks, 1)) - 1;
+        t[u] = make_double2(twg[2 * k], twg[2 * k + 1]);
+        q[u] = make_double4(X[(size_t)kk * m + r1], X[(size_t)cs * m + r1], X[(size_t)kk * m + rb],
+                            X[(size_t)cs * m + rb]);
+    }
+#pragma unroll
+    for (int u = 0; u < kE; u++) {
+        const int k = min((int)threadIdx.x + u * kBlock, N - 1);
+        const int kk = k <= N / 2 ? k : N - k;
+        const bool sin_ok = kk >= 1 && kk <= ks;
+        const double sg = k <= N / 2 ? 1.0 : -1.0;  // Hermitian: conj beyond N / 2
+        const double2 a = make_double2(q[u].x, sin_ok ? sg * q[u].y : 0.0);
+        double2 b = make_double2(q[u].z, sin_ok ? sg * q[u].w : 0.0);
+        if (r2 >= m) b = make_double2(0.0, 0.0);
+        v[u] = make_double2(a.x - b.y, -(a.y + b.x));  // conj(a + i b)
+    }
+#pragma unroll
+    for (int u = 0; u < kE; u++) {
+        const int k = threadIdx.x + u * kBlock;
+        if (k < N) {
+            tw[k] = t[u];
+            buf[k] = v[u];
+        }
     }
     __syncthreads();
     fft_lds<LMAX>(buf, tw, N);
