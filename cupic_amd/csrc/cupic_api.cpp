@@ -371,7 +371,6 @@ void seq_cc(const Target &T, int use_host_t, double t, int advance_t, bool clock
     {
         TimeScope ts(T, kTScan);
         launch_scan_a(T.L(kShapeCellBlk));
-        launch_scan_b(T.L(kShapeRunSp));
         launch_scan_c(T.L(kShapeCellBlk));
         launch_far_place(T.L(kShapeFar));
         launch_far_rank(T.L(kShapeFar));

@@ -83,7 +83,6 @@ void launch_inject(const Launch &L);
 void launch_far_place(const Launch &L);
 void launch_far_rank(const Launch &L);
 void launch_scan_a(const Launch &L);
-void launch_scan_b(const Launch &L);
 void launch_scan_c(const Launch &L);
 void launch_far_scatter(const Launch &L);
 void launch_finish(const Launch &L, int advance_t);

@@ -246,29 +246,10 @@ __global__ __launch_bounds__(kBlock) void k_scan_a(const RunDev *rds, const int 
     if (threadIdx.x == 0) R.blk[s][blk] = total;
 }
 
-// Exclusive scan of the block sums of one species; N_new = total.
-__global__ __launch_bounds__(kBlock) void k_scan_b(const RunDev *rds, const int *start,
-                                                   int nruns)
-{
-    RUN_PROLOGUE();
-    __shared__ int sh[kBlock / kWave];
-    int s = local;
-    if (s > 1) return;
-    int nblk = (R.ncells + kScanCells - 1) / kScanCells;
-    int carry = 0;
-    for (int b0 = 0; b0 < nblk; b0 += kBlock) {
-        int q = b0 + threadIdx.x;
-        int v = q < nblk ? R.blk[s][q] : 0;
-        int total;
-        int ex = block_excl_scan(v, sh, total);
-        if (q < nblk) R.blk[s][q] = carry + ex;
-        carry += total;
-    }
-    if (threadIdx.x == 0) st->N_new[s] = carry;
-}
-
 // New offsets and slot bases (slot-major): base[sl][k] = off[k] + arrivals
 // from slots before sl; slot 9 (far list) last.  Clears the far counters.
+// Each block starts from the sum of the block totals before it (at most a
+// few hundred, summed by the block itself); the last one also sets N_new.
 __global__ __launch_bounds__(kBlock) void k_scan_c(const RunDev *rds, const int *start,
                                                    int nruns)
 {
@@ -278,13 +259,17 @@ __global__ __launch_bounds__(kBlock) void k_scan_c(const RunDev *rds, const int 
     int nblk = (ncells + kScanCells - 1) / kScanCells;
     int s = local / nblk;
     int blk = local - s * nblk;
+    int pre = 0;
+    for (int q = threadIdx.x; q < blk; q += kBlock) pre += R.blk[s][q];
+    pre = block_sum(pre, sh);
+    __syncthreads();  // sh is reused below
     CUPIC_GLOBAL int *noff = gp(R.off[s][st->cur ^ 1]);
     CUPIC_GLOBAL int *base = gp(R.base[s]);
     const CUPIC_GLOBAL int *cnt = gp(R.cnt[s]);
     const CUPIC_GLOBAL int *tot = gp(R.tot[s]);
     CUPIC_GLOBAL int *farc = gp(R.far_cnt[s]);
     CUPIC_GLOBAL int *farcur = gp(R.far_cur[s]);
-    int carry = R.blk[s][blk];
+    int carry = pre;
     for (int u = 0; u < kScanCells / kBlock; u++) {
         int k = blk * kScanCells + u * kBlock + threadIdx.x;
         bool ok = k < ncells;
@@ -306,7 +291,10 @@ __global__ __launch_bounds__(kBlock) void k_scan_c(const RunDev *rds, const int 
         }
         carry += total;
     }
-    if (blk == nblk - 1 && threadIdx.x == 0) noff[ncells] = st->N_new[s];
+    if (blk == nblk - 1 && threadIdx.x == 0) {
+        st->N_new[s] = carry;
+        noff[ncells] = carry;
+    }
 }
 
 // Far list, after the scan: each entry takes a place in its cell's far
@@ -481,7 +469,6 @@ void launch_inject(const Launch &L) { LAUNCH_K(k_inject, L); }
 void launch_far_place(const Launch &L) { LAUNCH_K(k_far_place, L); }
 void launch_far_rank(const Launch &L) { LAUNCH_K(k_far_rank, L); }
 void launch_scan_a(const Launch &L) { LAUNCH_K(k_scan_a, L); }
-void launch_scan_b(const Launch &L) { LAUNCH_K(k_scan_b, L); }
 void launch_scan_c(const Launch &L) { LAUNCH_K(k_scan_c, L); }
 void launch_far_scatter(const Launch &L) { LAUNCH_K(k_far_scatter, L); }
 void launch_finish(const Launch &L, int advance_t) { LAUNCH_K(k_finish, L, advance_t); }

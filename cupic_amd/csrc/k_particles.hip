@@ -390,7 +390,7 @@ __device__ __forceinline__ void load_E_tile(const RunDev &R, float2 *Et, int j0,
 {
     int w = nc + 1;
     for (int q = lane; q < 2 * w; q += kWave) {
-        int r = q / w, c = col0 + (q - r * w);
+        int r = q >= w ? 1 : 0, c = col0 + (q - r * w);  // q < 2 w: two node rows
         if (c >= R.ncx) c -= R.ncx;
         Et[q] = ld_f2(R.E, (j0 + r) * R.ncx + c);
     }
@@ -460,7 +460,7 @@ __device__ __forceinline__ void count_task(const RunDev &R, RunState *st, const 
     {
         const int w = nc + 1;
         for (int q = lane; q < 2 * w; q += kWave) {
-            int r = q / w, c = col0 + (q - r * w);
+            int r = q >= w ? 1 : 0, c = col0 + (q - r * w);  // q < 2 w: two node rows
             if (c >= G.ncx) c -= G.ncx;
             float2 e = ld_f2(R.E, (j0 + r) * G.ncx + c);
             Et[q] = e;
@@ -652,9 +652,10 @@ __device__ __forceinline__ void count_task(const RunDev &R, RunState *st, const 
     wave_lds_sync();
     CUPIC_GLOBAL int *cnt = gp(R.cnt[s]);
     const int ncells = R.ncells;
-    for (int q = lane; q < nc * 9; q += kWave) {
-        int cl = q / 9, d = q - 9 * cl;
-        cnt[d * ncells + c0 + cl] = cntw[q];
+    // direction by direction: consecutive cells, coalesced (nc < kWave)
+    if (lane < nc) {
+#pragma unroll
+        for (int d = 0; d < 9; d++) cnt[d * ncells + c0 + lane] = cntw[__mul24(lane, 9) + d];
     }
     if (lane == 0) {
         if (probe_n) atomicAdd(&st->probe[s], probe_n);
