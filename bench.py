@@ -40,8 +40,9 @@ CONFIGS = {
 
 
 def workload_name(cfg, p, solver):
+    how = "FFT" if direct_uses_fft(cfg["ncx"], cfg["ncy"]) else "dense products"
     solve = ("Poisson to an FP64 residual < 1e-6 each step: the Jacobi fixed point solved "
-             "directly in FP64" if solver == "direct" else
+             f"directly in FP64 ({how})" if solver == "direct" else
              f"Jacobi FP64 residual {p.jacobi_tol:g} (max {p.jacobi_max_sweeps} sweeps)")
     return (f"{cfg['ncx']}x{cfg['ncy']} cells, {cfg['ppc']} ppc/species "
             f"(2x{p.per_row * p.ncy} particles), mi/me={cfg['mi']:g}, "
@@ -216,6 +217,15 @@ def draw_batched(n_runs, seed):
                         ppc=int(rng.integers(20, 101)), phi_p=float(rng.uniform(-10.0, 2.0)),
                         mi=100.0, kti=1.0, run_id=k))
     return out
+
+
+def direct_uses_fft(ncx, ncy):
+    """The library's choice for the direct solve: FFT kernels when ncx and
+    2 ncy are powers of two up to 4096 (and ncy >= 2), dense products
+    otherwise."""
+    def ok(n):
+        return 2 <= n <= 4096 and n & (n - 1) == 0
+    return ncy >= 2 and ok(ncx) and ok(2 * ncy)
 
 
 def batched_leg(cp, n_runs, steps, seq_steps, seed, fixed_tile=0, solver="direct"):
@@ -635,18 +645,34 @@ def main():
                            "bytes_per_particle": bytes_per_particle,
                            "avg_launch_ms": per[dom]}
     if kern and solver == "direct":
-        # the direct solve's four FP64 products: 2 x (m x ncx x ncx) for the
-        # DFT along x and 2 x (m x m x ncx) for the sine transform along y,
-        # m = ncy - 1, against the dense FP64 matrix peak
         solve_ms = out["kernels_ms_per_launch"]["solve"]
         m, nx = p.ncy - 1, p.ncx
-        flops = 2.0 * (2.0 * m * nx * nx + 2.0 * m * m * nx)
-        ach = flops / (solve_ms * 1e-3) / 1e12 if solve_ms > 0 else 0.0
-        out["solve_roofline"] = {"bound": "mfma", "achieved": ach, "peak": FP64_MFMA_PEAK,
-                                 "unit": "TFLOP/s", "frac": ach / FP64_MFMA_PEAK,
-                                 "flops_per_solve": flops, "avg_launch_ms": solve_ms,
-                                 "note": "whole solve (four products, slice sums, FP64 "
-                                         "residual) against the dense FP64 matrix peak"}
+        if direct_uses_fft(p.ncx, p.ncy):
+            # the FFT path: three transform kernels and the FP64 residual;
+            # algorithmic bytes: b and the Dirichlet rows in (8 B per node),
+            # the x spectra out and in (2 x 8), the y solve's result out and
+            # in (2 x 8), the FP64 and FP32 potential out (12), the residual
+            # reading b and the potential (16): 68 B per interior node
+            nbytes = 68.0 * m * nx
+            ach = nbytes / (solve_ms * 1e-3) if solve_ms > 0 else 0.0
+            out["solve_roofline"] = {"bound": "hbm", "achieved": ach / 1e9,
+                                     "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                                     "frac": ach / HBM_PEAK, "bytes_per_solve": nbytes,
+                                     "avg_launch_ms": solve_ms,
+                                     "note": "whole solve (set-up, three FFT kernels, FP64 "
+                                             "residual, end): six dependent launches of a "
+                                             "few microseconds each, launch-latency bound"}
+        else:
+            # the dense path's four FP64 products: 2 x (m x ncx x ncx) for the
+            # DFT along x and 2 x (m x m x ncx) for the sine transform along
+            # y, against the dense FP64 matrix peak
+            flops = 2.0 * (2.0 * m * nx * nx + 2.0 * m * m * nx)
+            ach = flops / (solve_ms * 1e-3) / 1e12 if solve_ms > 0 else 0.0
+            out["solve_roofline"] = {"bound": "mfma", "achieved": ach, "peak": FP64_MFMA_PEAK,
+                                     "unit": "TFLOP/s", "frac": ach / FP64_MFMA_PEAK,
+                                     "flops_per_solve": flops, "avg_launch_ms": solve_ms,
+                                     "note": "whole solve (four products, slice sums, FP64 "
+                                             "residual) against the dense FP64 matrix peak"}
     # measured HBM bytes of a whole step (every kernel, rocprofv3 counters,
     # copy-calibrated) over the measured step time
     tstep, src = profiled_traffic("step") if a.config == "prod" else (None, None)

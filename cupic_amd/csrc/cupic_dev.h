@@ -44,7 +44,7 @@ constexpr int kGM = 64;             // direct solve: GEMM output tile rows (4 wa
 constexpr int kGN = 64;             // direct solve: GEMM output tile columns (4 chains per wave)
 constexpr int kGK = 32;             // direct solve: GEMM k tile
 constexpr int kGSlices = 512;       // direct solve: workgroups a product aims for (k slices)
-constexpr int kGMaxSlices = 8;
+constexpr int kGMaxSlices = 2;   // at most: each slice adds a partial grid to write and sum
 // Direct solve by FFT: transform lengths (ncx along x, 2 ncy along y) that
 // are powers of two up to kFftL; two kernel instances, for lengths up to
 // kFftS and up to kFftL (LDS: two arrays of the length in complex doubles).
