@@ -45,7 +45,9 @@ extern "C" {
 #define CUPIC_JACOBI_RESIDUAL 1   /* FP32 sweeps, FP64 residual checked every check_every     */
 #define CUPIC_JACOBI_REFINE 2     /* FP64 residual + FP32 correction solve (refinement)        */
 #define CUPIC_POISSON_DIRECT 3    /* the system Jacobi converges to, solved directly in FP64:
-                                     DFT along periodic x, sine transform along y            */
+                                     DFT along periodic x, sine transform along y (FFT when
+                                     ncx and 2 ncy are powers of two up to 4096, else dense
+                                     FP64 products)                                          */
 
 /* quirk flags */
 #define CUPIC_QUIRK_Q7_REFERENCE_KICK 1u /* injection half-kick sign as committed (src/cc.cu:243-244) */
