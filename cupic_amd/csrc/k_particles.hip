@@ -373,14 +373,59 @@ __device__ __forceinline__ void tile_add_one(unsigned long long *tile, int TW, i
     atomicAdd(&tile[key + TW + 1], (unsigned long long)wx * (unsigned long long)wy);
 }
 
+// value + the DPP move kCtrl of it (0 where the source lane is out of its
+// row or the row is masked off): one step of a whole-wave scan, which needs
+// no lane conditions
+template <int kCtrl, int kRowMask>
+__device__ __forceinline__ unsigned dpp_add_u32(unsigned v)
+{
+    return v + (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, kCtrl, kRowMask, 0xF, true);
+}
+
+// The same step for a 64-bit value held as two 32-bit words: the low words
+// add with a carry into the high ones.
+template <int kCtrl, int kRowMask>
+__device__ __forceinline__ void dpp_add_u64w(unsigned &lo, unsigned &hi)
+{
+    const unsigned dl = (unsigned)__builtin_amdgcn_update_dpp(0, (int)lo, kCtrl, kRowMask, 0xF, true);
+    const unsigned dh = (unsigned)__builtin_amdgcn_update_dpp(0, (int)hi, kCtrl, kRowMask, 0xF, true);
+    const unsigned nl = lo + dl;
+    hi = hi + dh + (nl < lo ? 1u : 0u);
+    lo = nl;
+}
+
+// Inclusive scan over the whole wave (the total lands in lane 63): the steps
+// of seg_scan_u64 with one segment, where the zero fill of the DPP moves
+// already gives every lane what the conditions would.  a1's halves (sums
+// of at most 60000 weights of 2^16 or less) are added as 32-bit values.
+__device__ __forceinline__ void wave_scan_full(unsigned &sx, unsigned &sy,
+                                               unsigned long long &a2)
+{
+    unsigned lo = (unsigned)a2, hi = (unsigned)(a2 >> 32);
+    sx = dpp_add_u32<0x111, 0xF>(sx); sy = dpp_add_u32<0x111, 0xF>(sy);
+    dpp_add_u64w<0x111, 0xF>(lo, hi);
+    sx = dpp_add_u32<0x112, 0xF>(sx); sy = dpp_add_u32<0x112, 0xF>(sy);
+    dpp_add_u64w<0x112, 0xF>(lo, hi);
+    sx = dpp_add_u32<0x114, 0xF>(sx); sy = dpp_add_u32<0x114, 0xF>(sy);
+    dpp_add_u64w<0x114, 0xF>(lo, hi);
+    sx = dpp_add_u32<0x118, 0xF>(sx); sy = dpp_add_u32<0x118, 0xF>(sy);
+    dpp_add_u64w<0x118, 0xF>(lo, hi);
+    sx = dpp_add_u32<0x142, 0xA>(sx); sy = dpp_add_u32<0x142, 0xA>(sy);
+    dpp_add_u64w<0x142, 0xA>(lo, hi);
+    sx = dpp_add_u32<0x143, 0xC>(sx); sy = dpp_add_u32<0x143, 0xC>(sy);
+    dpp_add_u64w<0x143, 0xC>(lo, hi);
+    a2 = (unsigned long long)lo | ((unsigned long long)hi << 32);
+}
+
 // Reduces the per-lane stayer sums of old cell cl over the wave and adds
 // them to the tile cell of cl (stayers keep their cell).
 __device__ __forceinline__ void flush_stayers(unsigned long long *tile, int TW, int cl,
                                               unsigned long long a1, unsigned long long a2,
                                               int lane)
 {
-    a1 = seg_scan_u64(a1, lane, 0);
-    a2 = seg_scan_u64(a2, lane, 0);
+    unsigned sx = (unsigned)a1, sy = (unsigned)(a1 >> 32);
+    wave_scan_full(sx, sy, a2);
+    a1 = (unsigned long long)sx | ((unsigned long long)sy << 32);
     if (lane == kWave - 1 && (a2 >> 48) != 0ull) tile_add(tile, TW, 1 * TW + (cl + 1), a1, a2);
 }
 
