@@ -305,30 +305,6 @@ __device__ __forceinline__ unsigned long long dpp_u64(unsigned long long v)
     return ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo;
 }
 
-// Segmented inclusive prefix sum over the wave (segments are runs of lanes
-// that start at seg_start): Hillis-Steele within rows of 16 lanes with
-// row_shr, then across rows with row_bcast:15 and row_bcast:31; a lane adds
-// a partial sum only when its source lane lies in its own segment.
-__device__ __forceinline__ unsigned long long seg_scan_u64(unsigned long long v, int lane,
-                                                           int seg_start)
-{
-    const int r = lane & 15;
-    unsigned long long u;
-    u = dpp_u64<0x111, 0xF>(v);  // row_shr:1
-    if (r >= 1 && lane - 1 >= seg_start) v += u;
-    u = dpp_u64<0x112, 0xF>(v);  // row_shr:2
-    if (r >= 2 && lane - 2 >= seg_start) v += u;
-    u = dpp_u64<0x114, 0xF>(v);  // row_shr:4
-    if (r >= 4 && lane - 4 >= seg_start) v += u;
-    u = dpp_u64<0x118, 0xF>(v);  // row_shr:8
-    if (r >= 8 && lane - 8 >= seg_start) v += u;
-    u = dpp_u64<0x142, 0xA>(v);  // row_bcast:15 -> rows 1 and 3
-    if ((lane >> 4) & 1 && (lane & ~15) - 1 >= seg_start) v += u;
-    u = dpp_u64<0x143, 0xC>(v);  // row_bcast:31 -> rows 2 and 3
-    if (lane >= 32 && 31 >= seg_start) v += u;
-    return v;
-}
-
 // popcount of m over the lanes below this one
 __device__ __forceinline__ int mbcnt64(unsigned long long m)
 {
@@ -394,10 +370,12 @@ __device__ __forceinline__ void dpp_add_u64w(unsigned &lo, unsigned &hi)
     lo = nl;
 }
 
-// Inclusive scan over the whole wave (the total lands in lane 63): the steps
-// of seg_scan_u64 with one segment, where the zero fill of the DPP moves
-// already gives every lane what the conditions would.  a1's halves (sums
-// of at most 60000 weights of 2^16 or less) are added as 32-bit values.
+// Inclusive prefix sum over the whole wave (the total lands in lane 63):
+// Hillis-Steele within rows of 16 lanes with row_shr, then across rows with
+// row_bcast:15 and row_bcast:31.  The zero fill of the DPP moves gives each
+// lane exactly the partial sums it must add, so no lane conditions are
+// needed.  a1's halves (sums of at most 60000 weights of 2^16 or less) are
+// added as 32-bit values.
 __device__ __forceinline__ void wave_scan_full(unsigned &sx, unsigned &sy,
                                                unsigned long long &a2)
 {
@@ -863,17 +841,28 @@ __device__ __forceinline__ void scatter_task(const RunDev &R, RunState *st, cons
             if (stay) { acc1 += v1; acc2 += v2; }
             acc_n += ns;
         } else if (kSegStayers) {
-            // a chunk that spans cells: the running sums are flushed, and
-            // one segmented scan gives every cell segment its stayer sums,
-            // which the segment's last lane adds to its cell
+            // a chunk that spans cells: the running sums are flushed; the
+            // stayer sums of each cell segment are its last lane's prefix
+            // over the whole wave minus the prefix just before the segment
+            // (read from that lane), which the last lane adds to its cell
             if (acc_cl >= 0) flush_stayers(tile, TW, acc_cl, acc1, acc2, lane);
             acc1 = 0ull;
             acc2 = 0ull;
             acc_cl = -1;
             acc_n = 0;
             const int sstart = valid ? 63 - __clzll((long long)(heads & upto)) : lane;
-            const unsigned long long s1 = seg_scan_u64(stay ? v1 : 0ull, lane, sstart);
-            const unsigned long long s2 = seg_scan_u64(stay ? v2 : 0ull, lane, sstart);
+            unsigned px = stay ? (unsigned)v1 : 0u, py = stay ? (unsigned)(v1 >> 32) : 0u;
+            unsigned long long p2 = stay ? v2 : 0ull;
+            wave_scan_full(px, py, p2);
+            const int src = max(sstart - 1, 0);
+            unsigned bx = (unsigned)__shfl((int)px, src, kWave);
+            unsigned by = (unsigned)__shfl((int)py, src, kWave);
+            unsigned bl = (unsigned)__shfl((int)(unsigned)p2, src, kWave);
+            unsigned bh = (unsigned)__shfl((int)(unsigned)(p2 >> 32), src, kWave);
+            if (sstart == 0) bx = by = bl = bh = 0u;
+            const unsigned long long s1 =
+                (unsigned long long)(px - bx) | ((unsigned long long)(py - by) << 32);
+            const unsigned long long s2 = p2 - (((unsigned long long)bh << 32) | bl);
             const bool tail =
                 valid && (lane == kWave - 1 || i + 1 >= p1 || ((heads >> (lane + 1)) & 1ull));
             if (tail && cl >= 0 && cl < nc && (s2 >> 48) != 0ull)
