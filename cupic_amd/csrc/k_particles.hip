@@ -295,16 +295,6 @@ __global__ __launch_bounds__(kBlock) void k_half_kick(const RunDev *rds, const i
 
 // ---------------------------------------------------------------- push passes
 
-// One DPP move of a 64-bit value (two 32-bit halves); lanes whose source
-// is out of range or whose row is masked off read 0.
-template <int kCtrl, int kRowMask>
-__device__ __forceinline__ unsigned long long dpp_u64(unsigned long long v)
-{
-    int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)v, kCtrl, kRowMask, 0xF, true);
-    int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(v >> 32), kCtrl, kRowMask, 0xF, true);
-    return ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo;
-}
-
 // popcount of m over the lanes below this one
 __device__ __forceinline__ int mbcnt64(unsigned long long m)
 {
