@@ -119,3 +119,34 @@ def test_direct_solve_benchmark_grid(dev):
     assert resid < 1e-11
     assert err < 1e-11
     run.close()
+
+
+def test_direct_solve_batch_mixes_paths(dev):
+    """A batch holding runs of both FFT instances and of the dense products
+    steps each of them bit for bit as the run stepped alone."""
+    shapes = [dict(ncx=64, ncy=32, ppc=6), dict(ncx=2048, ncy=8, ppc=2, seed=5),
+              dict(ncx=40, ncy=20, ppc=6, seed=7), dict(ncx=16, ncy=1024, ppc=1, seed=9)]
+    alone, mixed = [], []
+    for kw in shapes:
+        p = cp.params(**kw, phi_p=-5.0, jacobi_mode=cp.POISSON_DIRECT, jacobi_tol=1e-11,
+                      jacobi_tol_init=1e-11)
+        for lst in (alone, mixed):
+            r = cp.Run(p)
+            r.init_sim()
+            lst.append(r)
+    for r in alone:
+        t = r.p.dt
+        for _ in range(4):
+            r.step(t)
+            t += r.p.dt
+    b = cp.Batch(mixed, graph=True)
+    b.step(4)
+    b.close()
+    for ra, rm, kw in zip(alone, mixed, shapes):
+        assert np.array_equal(ra.mesh(cp.MESH_PHI), rm.mesh(cp.MESH_PHI)), kw
+        for s in (0, 1):
+            for u, v in zip(ra.particles(s), rm.particles(s)):
+                assert np.array_equal(u, v), kw
+        say(f"{kw['ncx']}x{kw['ncy']}: batched == alone")
+    for r in alone + mixed:
+        r.close()
