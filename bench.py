@@ -70,18 +70,24 @@ def physical_cores():
     return (len(pairs) or None), len(cpus)
 
 
-def cpu_baseline(cfg, min_seconds, max_steps):
+def cpu_baseline(cfg, min_seconds, max_steps, solver="jacobi"):
     """The oracle's OpenMP FP64 cycle on the same workload (the CPU checker,
     never the measured path): full cycles after its own init, repeated until
     min_seconds of stepping have passed (at least 2, at most max_steps).  The
-    oracle solves Poisson with FP64 Jacobi sweeps (500 at most per step)."""
+    oracle solves Poisson as the GPU run does: directly in FP64 (FFT along
+    x, Thomas along y), or with FP64 Jacobi sweeps (500 at most per step)."""
     from oracle import oracle as O
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
     os.environ.setdefault("OMP_NUM_THREADS", str(threads))
     phys, ncpu = physical_cores()
     p = O.make_params(**cfg)
     sim = O.Sim(p, omp=True)
-    sim.set_solver(1, 1e-6, 1e-6, 0, 500, 50)
+    if solver == "direct":
+        sim.set_solver(3, 1e-6, 1e-6, 0, 0, 50)
+        how = "Poisson solved directly"
+    else:
+        sim.set_solver(1, 1e-6, 1e-6, 0, 500, 50)
+        how = "Jacobi capped at 500 sweeps"
     sim.init()
     t = p.dt
     n = 0
@@ -98,8 +104,8 @@ def cpu_baseline(cfg, min_seconds, max_steps):
             "physical_cores": phys, "cpus_available": ncpu, "steps": steps,
             "ms_per_step": 1e3 * el / steps,
             "sample": f"{steps} full cycles of the same workload after init "
-                      f"(FP64, OpenMP, {threads} threads on {phys} physical cores, Jacobi "
-                      f"capped at 500 sweeps), {el:.1f} s"}
+                      f"(FP64, OpenMP, {threads} threads on {phys} physical cores, {how}), "
+                      f"{el:.1f} s"}
 
 
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "cycle_1024x512_200ppc_hbm_traffic.md")
@@ -701,7 +707,7 @@ def main():
                                      20261203, a.serve_plain)
     if not a.no_cpu and a.cpu_max_steps > 0:
         print("# cpu baseline ...", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(cfg, a.cpu_seconds, a.cpu_max_steps)
+        out["cpu_baseline"] = cpu_baseline(cfg, a.cpu_seconds, a.cpu_max_steps, solver)
         if a.small_steps > 0:
             out["cpu_baseline_small_config"] = cpu_small(a.small_steps)
             g = out.get("small_config", {}).get("value")

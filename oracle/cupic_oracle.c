@@ -260,14 +260,164 @@ double orc_residual_max(int ncx, int ncy, double ds, double eps0, const double *
     return mx;
 }
 
+/* ---- mode 3: the fixed point of the sweeps, solved directly in FP64 ----
+ * The sweeps above converge to the solution of 4 phi - (W + E + N + S) =
+ * ds^2 rho / eps0 on rows 1..ncy-1, periodic in x, rows 0 and ncy fixed
+ * (src/mesh.cu:297-381 iterates towards it).  Along x a DFT diagonalises
+ * the periodic second difference (an FFT for power-of-two ncx, a direct
+ * O(ncx^2) DFT otherwise); per wavenumber k the rows then solve the
+ * tridiagonal system -x(j-1) + (2 + 4 sin^2(pi k / ncx)) x(j) - x(j+1) =
+ * r(j), zero beyond the interior, by the Thomas algorithm (real matrix,
+ * complex right-hand side); an inverse DFT gives the rows back. */
+
+/* e^(sign 2 pi i t / n), t = 0 .. n-1 */
+static void dft_table(int n, int sign, double *twr, double *twi)
+{
+    for (int t = 0; t < n; t++) {
+        double a = 2.0 * ORC_PI * (double)t / (double)n;
+        twr[t] = cos(a);
+        twi[t] = sign * sin(a);
+    }
+}
+
+/* in place, unnormalised: X_k = sum_t x_t tw[(t k) mod n] */
+static void dft_row(int n, double *re, double *im, const double *twr, const double *twi,
+                    double *sr, double *si)
+{
+    if ((n & (n - 1)) == 0) {
+        /* iterative radix-2: bit reversal, then butterflies of growing span */
+        for (int i = 1, j = 0; i < n; i++) {
+            int bit = n >> 1;
+            for (; j & bit; bit >>= 1) j ^= bit;
+            j ^= bit;
+            if (i < j) {
+                double t = re[i]; re[i] = re[j]; re[j] = t;
+                t = im[i]; im[i] = im[j]; im[j] = t;
+            }
+        }
+        for (int len = 2; len <= n; len <<= 1) {
+            int h = len >> 1, step = n / len;
+            for (int j = 0; j < h; j++) {
+                double wr = twr[j * step], wi = twi[j * step];
+                for (int i = j; i < n; i += len) {
+                    int k = i + h;
+                    double tr = re[k] * wr - im[k] * wi, ti = re[k] * wi + im[k] * wr;
+                    re[k] = re[i] - tr;
+                    im[k] = im[i] - ti;
+                    re[i] += tr;
+                    im[i] += ti;
+                }
+            }
+        }
+        return;
+    }
+    for (int k = 0; k < n; k++) {
+        double ar = 0.0, ai = 0.0;
+        for (int t = 0; t < n; t++) {
+            int q = (int)(((int64_t)t * k) % n);
+            ar += re[t] * twr[q] - im[t] * twi[q];
+            ai += re[t] * twi[q] + im[t] * twr[q];
+        }
+        sr[k] = ar;
+        si[k] = ai;
+    }
+    memcpy(re, sr, (size_t)n * sizeof(double));
+    memcpy(im, si, (size_t)n * sizeof(double));
+}
+
+static void direct_solve(int ncx, int ncy, double ds, double eps0, const double *rho,
+                         double *phi)
+{
+    const int m = ncy - 1, n = ncx;
+    const double c = ds * ds;
+    double *xr = (double *)malloc((size_t)m * n * sizeof(double));
+    double *xi = (double *)malloc((size_t)m * n * sizeof(double));
+    double *fr = (double *)malloc((size_t)n * sizeof(double));
+    double *fi = (double *)malloc((size_t)n * sizeof(double));
+    double *br = (double *)malloc((size_t)n * sizeof(double));
+    double *bi = (double *)malloc((size_t)n * sizeof(double));
+    dft_table(n, -1, fr, fi);
+    dft_table(n, +1, br, bi);
+    /* rows 1..m of the right-hand side, the Dirichlet rows moved into the
+     * first and last of them, forward DFT */
+#pragma omp parallel
+    {
+        double *sr = (double *)malloc((size_t)n * sizeof(double));
+        double *si = (double *)malloc((size_t)n * sizeof(double));
+#pragma omp for schedule(static)
+        for (int r = 0; r < m; r++) {
+            double *re = xr + (int64_t)r * n, *im = xi + (int64_t)r * n;
+            for (int i = 0; i < n; i++) {
+                double v = c * rho[(int64_t)(r + 1) * n + i] / eps0;
+                if (r == 0) v += phi[i];
+                if (r == m - 1) v += phi[(int64_t)ncy * n + i];
+                re[i] = v;
+                im[i] = 0.0;
+            }
+            dft_row(n, re, im, fr, fi, sr, si);
+        }
+        free(sr);
+        free(si);
+    }
+    /* per wavenumber, the tridiagonal system along y (Thomas) */
+#pragma omp parallel
+    {
+        double *cp = (double *)malloc((size_t)m * sizeof(double));
+#pragma omp for schedule(static)
+        for (int k = 0; k < n; k++) {
+            double sk = sin(ORC_PI * (double)k / (double)n);
+            double d = 2.0 + 4.0 * sk * sk;
+            double den = d;
+            cp[0] = -1.0 / den;
+            xr[k] /= den;
+            xi[k] /= den;
+            for (int j = 1; j < m; j++) {
+                den = d + cp[j - 1];
+                cp[j] = -1.0 / den;
+                int64_t e = (int64_t)j * n + k, f = e - n;
+                xr[e] = (xr[e] + xr[f]) / den;
+                xi[e] = (xi[e] + xi[f]) / den;
+            }
+            for (int j = m - 2; j >= 0; j--) {
+                int64_t e = (int64_t)j * n + k, f = e + n;
+                xr[e] -= cp[j] * xr[f];
+                xi[e] -= cp[j] * xi[f];
+            }
+        }
+        free(cp);
+    }
+    /* inverse DFT into rows 1..m */
+#pragma omp parallel
+    {
+        double *sr = (double *)malloc((size_t)n * sizeof(double));
+        double *si = (double *)malloc((size_t)n * sizeof(double));
+#pragma omp for schedule(static)
+        for (int r = 0; r < m; r++) {
+            double *re = xr + (int64_t)r * n, *im = xi + (int64_t)r * n;
+            dft_row(n, re, im, br, bi, sr, si);
+            for (int i = 0; i < n; i++) phi[(int64_t)(r + 1) * n + i] = re[i] / (double)n;
+        }
+        free(sr);
+        free(si);
+    }
+    free(xr); free(xi); free(fr); free(fi); free(br); free(bi);
+}
+
 /* mode 0: the reference rule, src/mesh.cu:66-67, 94-114 -- sweep until
  * max|dphi| < tol and at least min_sweeps sweeps ran (the reference uses
  * max(nnx,nny)+1); capped at max_sweeps (the reference has no cap).
- * modes 1, 2: residual checked before every block of check_every sweeps. */
+ * modes 1, 2: residual checked before every block of check_every sweeps.
+ * mode 3: the direct solve above (no sweeps); the residual after it. */
 int orc_poisson_solve(int ncx, int ncy, double ds, double eps0, const double *rho,
                       double *phi, int mode, double tol, int min_sweeps, int max_sweeps,
                       int check_every, double *resid_out)
 {
+    if (mode == 3) {
+        direct_solve(ncx, ncy, ds, eps0, rho, phi);
+        double res = orc_residual_max(ncx, ncy, ds, eps0, rho, phi);
+        if (resid_out) *resid_out = res;
+        return 0;
+    }
     int64_t nodes = (int64_t)ncx * (ncy + 1);
     double *work = (double *)malloc((size_t)nodes * sizeof(double));
     memcpy(work, phi, (size_t)nodes * sizeof(double));

@@ -185,6 +185,44 @@ def test_manufactured_solution_and_scipy_direct_solve():
     assert np.abs(phi.reshape(sh) - star).max() < 1e-9
 
 
+@pytest.mark.parametrize("ncx,ncy", [(16, 8), (12, 6), (37, 9), (64, 2), (5, 4), (128, 64)])
+def test_direct_mode_matches_sparse_solve(ncx, ncy):
+    """mode 3 (FFT or direct DFT along x, Thomas along y) against scipy's
+    sparse factorisation of the same 5-point system, and against the
+    manufactured solution, for even, odd, power-of-two and other ncx."""
+    p = small(ncx=ncx, ncy=ncy, phi_p=-5.0)
+    sh = (p.ncy + 1, p.ncx)
+    r = np.random.default_rng(ncx * 100 + ncy)
+    rho = np.zeros(sh)
+    rho[1:-1] = r.normal(size=(p.ncy - 1, p.ncx)) * 5
+    phi = np.zeros(O.nodes(p))
+    L.orc_phi_linear(p.ncx, p.ncy, p.phi_p, phi)
+    phi.reshape(sh)[0] += r.normal(size=p.ncx)   # non-uniform Dirichlet rows
+    phi.reshape(sh)[-1] += r.normal(size=p.ncx)
+    want = solve_direct(p, rho, phi.reshape(sh).copy())
+    import ctypes as C
+    res = C.c_double()
+    s = L.orc_poisson_solve(p.ncx, p.ncy, p.ds, p.eps0, rho.ravel().copy(), phi, 3, 0.0, 0, 0,
+                            50, C.byref(res))
+    assert s == 0
+    got = phi.reshape(sh)
+    assert np.abs(got - want).max() < 1e-10 * max(1.0, np.abs(want).max())
+    assert res.value < 1e-10
+    # the manufactured solution of test_manufactured_solution_and_scipy_direct_solve
+    jj, ii = np.mgrid[0:p.ncy + 1, 0:p.ncx]
+    star = np.sin(2 * np.pi * ii / p.ncx) * np.sin(np.pi * jj / p.ncy) \
+        + (1 - jj / p.ncy) * p.phi_p
+    lap = (np.roll(star, 1, 1) + np.roll(star, -1, 1) - 4 * star)
+    lap[1:-1] += star[:-2] + star[2:]
+    rho2 = np.zeros(sh)
+    rho2[1:-1] = -lap[1:-1] * p.eps0 / p.ds**2
+    phi2 = np.zeros(O.nodes(p))
+    L.orc_phi_linear(p.ncx, p.ncy, p.phi_p, phi2)
+    L.orc_poisson_solve(p.ncx, p.ncy, p.ds, p.eps0, rho2.ravel().copy(), phi2, 3, 0.0, 0, 0, 50,
+                        C.byref(res))
+    assert np.abs(phi2.reshape(sh) - star).max() < 1e-11
+
+
 def test_reference_stopping_rule_min_sweeps():
     """mode 0: at least min_sweeps sweeps, then until max|dphi| < tol
     (src/mesh.cu:66-67, 94-113)."""
