@@ -459,25 +459,56 @@ __device__ __forceinline__ void count_task(const RunDev &R, RunState *st, const 
     const int far_cap = R.far_cap;
     int *const far_cnt = R.far_cnt[s];
     FarRec *const farl = R.far[s];
-    const int cur = st->cur;
-    const CUPIC_GLOBAL int *off = gp(R.off[s][cur]);
-    const int c0 = j0 * R.ncx + col0;
-    const int p0 = off[c0], p1 = off[c0 + nc];
-    const float4 *PIN = R.part[s][cur];
-    const CUPIC_GLOBAL unsigned char *HIN = gp(R.hint[s][cur]);
     const float qdtm = R.sp[s].qdtm;
     const int mode = R.hint_mode;
+    const int c0 = j0 * R.ncx + col0;
+    // The strip's E tile (2 (nc + 1) nodes, two per lane) does not depend on
+    // the buffer parity: its loads go out first, beside the parity's, and
+    // the first hint words follow the strip's offsets (scalar loads, so that
+    // they do not wait for the E tile).
+    static_assert(kCountCells + 1 <= kWave, "prologue staging");
+    const int w = nc + 1;
+    float2 e2[2];
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        const int q = k * kWave + lane;
+        e2[k] = make_float2(0.0f, 0.0f);
+        if (q < 2 * w) {
+            const int r = q >= w ? 1 : 0;  // q < 2 w: two node rows
+            int c = col0 + (q - r * w);
+            if (c >= G.ncx) c -= G.ncx;
+            e2[k] = ld_f2(R.E, (j0 + r) * G.ncx + c);
+        }
+    }
+    const int cur = __builtin_amdgcn_readfirstlane(gp(st)->cur);
+    const CUPIC_GLOBAL int *off = gp(cur ? R.off[s][1] : R.off[s][0]);
+    const int p0u = __builtin_amdgcn_readfirstlane(off[c0]);
+    const int p1u = __builtin_amdgcn_readfirstlane(off[c0 + nc]);
+    const float4 *PIN = cur ? R.part[s][1] : R.part[s][0];
+    const CUPIC_GLOBAL unsigned char *HIN = gp(cur ? R.hint[s][1] : R.hint[s][0]);
+    const CUPIC_GLOBAL unsigned *HW = (const CUPIC_GLOBAL unsigned *)HIN;
+    // Hints are read as words of four slots, lane l covering slots
+    // w + 4l .. w + 4l + 3 of the 256-slot window starting at w; the words of
+    // the next window are in flight while one is processed.  Every lane
+    // loads a clamped, in-bounds word whatever the mode, so that the count
+    // of loads in flight is the same on every path and the wait for one
+    // window does not wait for the next; bytes of slots outside the strip,
+    // and all bytes unless the mode trusts hints, are ignored.
+    auto load_word = [&](int w0) -> unsigned {
+        return HW[max(0, min(w0 + 4 * lane, p1u - 1)) >> 2];
+    };
+    const int a0 = p0u & ~3;
+    unsigned wa = load_word(a0);
+    // lane k <= nc: first slot of strip cell k
+    const int bv = lane <= nc ? off[c0 + lane] : 0x7FFFFFFF;
     // E tile of the strip and its largest component
     float em = 0.0f;
     bool fin = true;
-    {
-        const int w = nc + 1;
-        for (int q = lane; q < 2 * w; q += kWave) {
-            int r = q >= w ? 1 : 0, c = col0 + (q - r * w);  // q < 2 w: two node rows
-            if (c >= G.ncx) c -= G.ncx;
-            float2 e = ld_f2(R.E, (j0 + r) * G.ncx + c);
-            Et[q] = e;
-            float ax = fabsf(e.x), ay = fabsf(e.y);
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        if (k * kWave + lane < 2 * w) {
+            Et[k * kWave + lane] = e2[k];
+            const float ax = fabsf(e2[k].x), ay = fabsf(e2[k].y);
             fin = fin && ax < INFINITY && ay < INFINITY;  // false for NaN
             em = fmaxf(em, fmaxf(ax, ay));
         }
@@ -490,7 +521,7 @@ __device__ __forceinline__ void count_task(const RunDev &R, RunState *st, const 
     const unsigned code_add = mode == kHintOn ? (unsigned)(16 - c_need) * 0x01010101u : 0u;
     for (int q = lane; q < nc * 9; q += kWave) cntw[q] = 0;
     for (int q = lane; q < nc; q += kWave) gonew[q] = 0;
-    for (int q = lane; q <= nc; q += kWave) bnd[q] = off[c0 + q];
+    if (lane <= nc) bnd[lane] = bv;
     wave_lds_sync();
     const unsigned long long lt = (1ull << lane) - 1ull;
     unsigned long long probe_n = 0, top_n = 0, full_n = 0;
@@ -565,14 +596,6 @@ __device__ __forceinline__ void count_task(const RunDev &R, RunState *st, const 
         }
     };
 
-    // Hints are read as words of four slots, lane l covering slots
-    // w + 4l .. w + 4l + 3 of the 256-slot window starting at w; the words of
-    // the next window are in flight while one is processed.
-    const int p0u = __builtin_amdgcn_readfirstlane(p0), p1u = __builtin_amdgcn_readfirstlane(p1);
-    // lane k: first slot of strip cell k (from LDS, so that no wait for HBM
-    // loads in the loop waits on the hint prefetch too)
-    const int bv = lane <= nc ? bnd[lane] : 0x7FFFFFFF;
-    const CUPIC_GLOBAL unsigned *HW = (const CUPIC_GLOBAL unsigned *)HIN;
     int cb = 0;  // strip cell of the window's first slot
     int qh = 0;  // queue: ring of kQueue slots, head qh, qn entries
     int qn = 0;
@@ -625,15 +648,6 @@ __device__ __forceinline__ void count_task(const RunDev &R, RunState *st, const 
             qn -= kWave;
         }
     };
-    // every lane loads a clamped, in-bounds word whatever the mode, so that
-    // the count of loads in flight is the same on every path and the wait
-    // for one window does not wait for the next; bytes of slots outside the
-    // strip, and all bytes unless the mode trusts hints, are ignored
-    auto load_word = [&](int w0) -> unsigned {
-        return HW[max(0, min(w0 + 4 * lane, p1u - 1)) >> 2];
-    };
-    const int a0 = p0u & ~3;
-    unsigned wa = load_word(a0);
     for (int w0 = a0; w0 < p1u; w0 += 8 * kWave) {
         const unsigned wb = load_word(w0 + 4 * kWave);
         window(w0, wa);
@@ -691,36 +705,55 @@ __device__ __forceinline__ void scatter_task(const RunDev &R, RunState *st, cons
     const int s = T.s, j0 = T.j0, col0 = T.col0, nc = T.nc;
     const Geo G = geo_of(R);
     const int cap = R.cap;
-    const int cur = st->cur;
-    const CUPIC_GLOBAL int *off = gp(R.off[s][cur]);
-    const int c0 = j0 * R.ncx + col0;
-    const int p0 = off[c0], p1 = off[c0 + nc];
-    const float4 *PIN = R.part[s][cur];
-    float4 *POUT = R.part[s][cur ^ 1];
-    CUPIC_GLOBAL unsigned char *HOUT = gp(R.hint[s][cur ^ 1]);
     const int ncells = R.ncells;
     const float2 *E = R.E;
     const float qdtm = R.sp[s].qdtm;
     const int TW = nc + 3;
-    load_E_tile(R, Et, j0, col0, nc, lane);
-    {
-        // counters start at the base of the slot each (cell, direction) fills
-        const CUPIC_GLOBAL int *base = gp(R.base[s]);
-        for (int q = lane; q < nc * 9; q += kWave) {
-            int cl = q / 9, d = q - 9 * cl;
-            int di = d % 3 - 1, dj = d / 3 - 1;
-            int jj = j0 + dj;
-            int v = 0;
-            if (jj >= 0 && jj < R.ncy) {
-                int ii = col0 + cl + di;
-                if (ii < 0) ii += R.ncx;
-                if (ii >= R.ncx) ii -= R.ncx;
-                v = base[(8 - d) * ncells + jj * R.ncx + ii];
-            }
-            cntw[q] = v;
-        }
-        for (int q = lane; q < 4 * TW; q += kWave) tile[q] = 0ull;
+    const int c0 = j0 * R.ncx + col0;
+    // The strip's E tile (2 (nc + 1) nodes) and slot bases (9 nc) do not
+    // depend on the buffer parity: their loads go out first, beside the
+    // parity's, and the first chunk's records follow the strip's offsets,
+    // so that a task waits for three round trips, not six, before its first
+    // particle.  Counters start at the base of the slot each (cell,
+    // direction) fills.
+    static_assert(2 * (kCmax + 1) <= kWave && 9 * kCmax <= 3 * kWave, "prologue staging");
+    float2 ev = make_float2(0.0f, 0.0f);
+    if (lane < 2 * (nc + 1)) {
+        const int r = lane > nc ? 1 : 0;
+        int c = col0 + (lane - r * (nc + 1));
+        if (c >= R.ncx) c -= R.ncx;
+        ev = ld_f2(E, (j0 + r) * R.ncx + c);
     }
+    int bv[3];
+    {
+        const CUPIC_GLOBAL int *base = gp(R.base[s]);
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const int q = k * kWave + lane;
+            const int cl = q / 9, d = q - 9 * cl;
+            const int di = d % 3 - 1, jj = j0 + d / 3 - 1;
+            int ii = col0 + cl + di;
+            if (ii < 0) ii += R.ncx;
+            if (ii >= R.ncx) ii -= R.ncx;
+            bv[k] = 0;
+            if (q < 9 * nc && jj >= 0 && jj < R.ncy)
+                bv[k] = base[(8 - d) * ncells + jj * R.ncx + ii];
+        }
+    }
+    const int cur = __builtin_amdgcn_readfirstlane(gp(st)->cur);
+    const CUPIC_GLOBAL int *off = gp(cur ? R.off[s][1] : R.off[s][0]);
+    const int p0 = __builtin_amdgcn_readfirstlane(off[c0]);
+    const int p1 = __builtin_amdgcn_readfirstlane(off[c0 + nc]);
+    const float4 *PIN = cur ? R.part[s][1] : R.part[s][0];
+    float4 *POUT = cur ? R.part[s][0] : R.part[s][1];
+    CUPIC_GLOBAL unsigned char *HOUT = gp(cur ? R.hint[s][0] : R.hint[s][1]);
+    v4f nxt = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+    if (p0 + lane < p1) nxt = ld4(PIN, p0 + lane);
+    if (lane < 2 * (nc + 1)) Et[lane] = ev;
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+        if (k * kWave + lane < 9 * nc) cntw[k * kWave + lane] = bv[k];
+    for (int q = lane; q < 4 * TW; q += kWave) tile[q] = 0ull;
     wave_lds_sync();
     const unsigned long long lt = (1ull << lane) - 1ull;
     const unsigned long long upto = lt | (1ull << lane);
@@ -731,8 +764,6 @@ __device__ __forceinline__ void scatter_task(const RunDev &R, RunState *st, cons
     // are issued during the next chunk, after its prefetch: loads and stores
     // share one wait counter here, so the wait for a prefetch also waits for
     // every store issued before it, and stores one chunk old have completed.
-    v4f nxt = v4f{0.0f, 0.0f, 0.0f, 0.0f};
-    if (p0 + lane < p1) nxt = ld4(PIN, p0 + lane);
     v4f prec = v4f{0.0f, 0.0f, 0.0f, 0.0f};  // the previous chunk's store
     int pdest = -1;
     unsigned phint = 0u;
