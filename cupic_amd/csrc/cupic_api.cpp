@@ -223,11 +223,14 @@ struct TimeScope {
     }
 };
 
+// begun: the solve was started by the charge conversion (launch_rho_convert
+// with begin set, use_args 0), so no k_solver_begin launch.
 void seq_solve(const Target &T, int use_args, int mode, double tol, int min_sweeps,
-               int max_sweeps, int check)
+               int max_sweeps, int check, bool begun = false)
 {
     TimeScope ts(T, kTSolve);
-    launch_solver_begin(T.L(kShapeRun), use_args, mode, tol, min_sweeps, max_sweeps, check);
+    if (!begun)
+        launch_solver_begin(T.L(kShapeRun), use_args, mode, tol, min_sweeps, max_sweeps, check);
     Launch N = T.L(kShapeNodes);
     if (mode == 0) {
         for (int s = 0; s < max_sweeps; s++) launch_jacobi_sweep(N, s);
@@ -247,8 +250,7 @@ void seq_solve(const Target &T, int use_args, int mode, double tol, int min_swee
             launch_dft_gemm(T.L(step == 0 || step == 3 ? kShapeDftGemmX : kShapeDftGemmY), step);
             launch_dft_reduce(T.L(kShapeDftReduce), step);
         }
-        launch_residual(T.L(kShapeResid), 0, 3, 0);
-        launch_solver_end(T.L(kShapeRun), 0, 0);
+        launch_residual(T.L(kShapeResid), 0, 3, 0, 1);  // its last block ends the solve
         return;
     }
     const bool narrow = T.nblk[kShapeJacBlk] >= kJacNarrowMinBlocks;
@@ -272,7 +274,7 @@ void seq_solve(const Target &T, int use_args, int mode, double tol, int min_swee
         const bool through = !narrow;
         const int step = std::min(per, check);  // so at most one check per launch
         const bool fuse0 = through && check > per && max_sweeps > per;
-        if (!fuse0) launch_residual(T.L(kShapeResid), 0, mode, 0);
+        if (!fuse0) launch_residual(T.L(kShapeResid), 0, mode, 0, 0);
         int cslot = fuse0 ? -1 : 0, cat = 0, cpar = 0;  // the check the next launch reads
         while (s < max_sweeps) {
             int nsw = std::min(step, max_sweeps - s);
@@ -305,7 +307,7 @@ void seq_solve(const Target &T, int use_args, int mode, double tol, int min_swee
     }
     for (;;) {
         if (c > 0) launch_ir_update(N, dpar);
-        launch_residual(T.L(kShapeResid), c, mode, 0);
+        launch_residual(T.L(kShapeResid), c, mode, 0, 0);
         if (s >= max_sweeps) break;
         int k = std::min(check, max_sweeps - s);
         int dl = 0;
@@ -340,10 +342,10 @@ int set_global_error(int code, const char *msg) { return set_err(nullptr, code, 
 
 namespace {
 
-void seq_rho(const Target &T)
+void seq_rho(const Target &T, int begin_solve = 0)
 {
     TimeScope ts(T, kTDeposit);
-    launch_rho_convert(T.L(kShapeNodes));
+    launch_rho_convert(T.L(kShapeNodes), begin_solve);
 }
 
 void seq_field(const Target &T)
@@ -381,8 +383,7 @@ void seq_cc(const Target &T, int use_host_t, double t, int advance_t, bool clock
     }
     {
         TimeScope ts(T, kTFinish);
-        launch_far_scatter(T.L(kShapeFar));
-        launch_finish(T.L(kShapeRun), advance_t);
+        launch_far_scatter(T.L(kShapeFar), advance_t);
     }
 }
 
@@ -509,6 +510,23 @@ void to_reference(const std::vector<T> &in, int ncx, int ncy, T *out)
         for (int i = 0; i < ncx; i++) out[(size_t)j * nnx + i] = in[(size_t)j * ncx + i];
         out[(size_t)j * nnx + ncx] = in[(size_t)j * ncx];
     }
+}
+
+// charge_deposition; begin_solve: the conversion also starts the step's
+// solve (a seq_solve with begun set must follow)
+int charge_deposition(cupic_run *r, int begin_solve)
+{
+    if (!r) return set_err(nullptr, CUPIC_ERR_ARG, "null run");
+    if (r->moved) return set_err(&r->err, CUPIC_ERR_STATE, "charge_deposition between mover and cc");
+    Target T = target_of(r);
+    if (!r->acc_valid) {
+        HIPTRY(&r->err, hipMemsetAsync(r->h.acc, 0, (size_t)r->h.nnodes * 8, r->stream));
+        TimeScope ts(T, kTDeposit);
+        launch_deposit(T.L(kShapeTasks));
+    }
+    seq_rho(T, begin_solve);
+    r->acc_valid = false;
+    return check_launch(r, "charge_deposition");
 }
 
 }  // namespace
@@ -693,7 +711,7 @@ int cupic_run_footprint(const cupic_params *pin, int64_t capacity_hint, int64_t 
     b += 2 * 2 * 4 * z.cap * 4;                  // particles
     b += 2 * 2 * ((z.cap + 255) & ~(int64_t)255);  // push hints
     b += 2 * 2 * (z.ncells + 1) * 4;             // offsets
-    b += 2 * z.ncells * (9 + 10 + 1 + 1 + 1) * 4;  // cnt, base, far_cnt, far_cur, tot
+    b += 2 * z.ncells * (9 + 10 + 1 + 1) * 4;  // cnt, base, far_cnt, far_cur
     b += 2 * z.far_cap * (int64_t)sizeof(FarRec) + 2 * z.max_inject * 8;
     b += z.nnodes * 64;                          // grids
     b += (int64_t)(p.jacobi_max_sweeps + 2) * 12 * kRedSlots;
@@ -785,7 +803,6 @@ int cupic_run_create(const cupic_params *pin, int64_t capacity_hint, cupic_run *
         h.far_cur[s] = dev_alloc<int>(r, z.ncells, &st);
         h.far[s] = dev_alloc<FarRec>(r, z.far_cap, &st);
         h.inj_tin[s] = dev_alloc<double>(r, z.max_inject, &st);
-        h.tot[s] = dev_alloc<int>(r, z.ncells, &st);
         h.blk[s] = dev_alloc<int>(r, z.nblk_scan, &st);
     }
     h.acc = dev_alloc<unsigned long long>(r, z.nnodes, &st);
@@ -900,20 +917,7 @@ int cupic_create_particles(cupic_run *r)
     return check_launch(r, "create_particles");
 }
 
-int cupic_charge_deposition(cupic_run *r)
-{
-    if (!r) return set_err(nullptr, CUPIC_ERR_ARG, "null run");
-    if (r->moved) return set_err(&r->err, CUPIC_ERR_STATE, "charge_deposition between mover and cc");
-    Target T = target_of(r);
-    if (!r->acc_valid) {
-        HIPTRY(&r->err, hipMemsetAsync(r->h.acc, 0, (size_t)r->h.nnodes * 8, r->stream));
-        TimeScope ts(T, kTDeposit);
-        launch_deposit(T.L(kShapeTasks));
-    }
-    seq_rho(T);
-    r->acc_valid = false;
-    return check_launch(r, "charge_deposition");
-}
+int cupic_charge_deposition(cupic_run *r) { return charge_deposition(r, 0); }
 
 int cupic_initialize_mesh(cupic_run *r)
 {
@@ -1005,14 +1009,14 @@ int cupic_step(cupic_run *r, double t)
 {
     if (!r) return set_err(nullptr, CUPIC_ERR_ARG, "null run");
     if (r->moved) return set_err(&r->err, CUPIC_ERR_STATE, "step between mover and cc");
-    int st = cupic_charge_deposition(r);
+    int st = charge_deposition(r, 1);
     if (!st) {
         seq_clock_fork(target_of(r), 1, t);
         r->clock_forked = true;
     }
     if (!st) {
         seq_solve(target_of(r), 0, r->p.jacobi_mode, 0, 0, r->p.jacobi_max_sweeps,
-                  r->p.jacobi_check_every);
+                  r->p.jacobi_check_every, true);
         seq_field(target_of(r));
         st = check_launch(r, "solve/field");
     }
@@ -1536,9 +1540,9 @@ Target batch_target(cupic_batch *b)
 // One step of every run in the batch; t lives on the device and advances.
 void batch_step_sequence(cupic_batch *b, const Target &T)
 {
-    launch_rho_convert(T.L(kShapeNodes));
+    launch_rho_convert(T.L(kShapeNodes), 1);
     seq_clock_fork(T, 0, 0.0);
-    seq_solve(T, 0, b->mode, 0, 0, b->max_sweeps, b->check);
+    seq_solve(T, 0, b->mode, 0, 0, b->max_sweeps, b->check, true);
     launch_field(T.L(kShapeNodes));
     launch_push_rank(T.L(kShapeCountTasks));
     seq_cc(T, 0, 0.0, 1, true);

@@ -39,7 +39,7 @@ constexpr int kCountCells = 32;     // cells per wave strip of the counting pass
 constexpr int kScanCells = 1024;    // cells per scan block
 constexpr int kRedSlots = 16;       // atomic slots per reduced quantity
 constexpr int kResNodes = 8;        // FP64 residual: nodes per thread
-constexpr int kFarBlocks = 16;      // far-list kernels: at most this many blocks per species
+constexpr int kFarBlocks = 64;      // far-list kernels: at most this many blocks per species
 constexpr int kGM = 64;             // direct solve: GEMM output tile rows (4 waves x 16)
 constexpr int kGN = 64;             // direct solve: GEMM output tile columns (4 chains per wave)
 constexpr int kGK = 32;             // direct solve: GEMM k tile
@@ -121,6 +121,8 @@ struct RunState {
     unsigned long long particle_steps;  // sum over steps of the particles moved
     double probe_charge_total;
     unsigned int err;               // error bits, see kErr*
+    unsigned int fin_ticket;        // k_far_scatter blocks done this step (the last ends it)
+    unsigned int solve_ticket;      // k_residual blocks done (end: the last ends the solve)
 };
 
 enum : unsigned int {
@@ -171,7 +173,6 @@ struct RunDev {
     int *far_cur[2];                // ncells, placement cursors
     FarRec *far[2];                 // far_cap
     double *inj_tin[2];             // max_inject
-    int *tot[2];                    // ncells
     int *blk[2];                    // scan block sums / offsets
     unsigned long long *acc;        // nnodes, int64 charge
     float *rho;                     // nnodes

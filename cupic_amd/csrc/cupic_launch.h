@@ -49,14 +49,16 @@ int shape_blocks(const RunDev &h, Shape s);
 
 // mesh
 void launch_phi_linear(const Launch &L);
-void launch_rho_convert(const Launch &L);
+// begin: block 0 also starts the solve with the run's step settings
+void launch_rho_convert(const Launch &L, int begin);
 void launch_solver_begin(const Launch &L, int use_args, int mode, double tol, int min_sweeps,
                          int max_sweeps, int check_every);
 void launch_jacobi_sweep(const Launch &L, int s);
 void launch_jacobi_block(const Launch &L, int halo, int s0, int nsw, int mode, int first,
                          int par, int cslot, int cat, int cpar, int rslot, int rat);
 void launch_ir_update(const Launch &L, int par);
-void launch_residual(const Launch &L, int c, int mode, int par);
+// end: the last block also ends the solve at check c (as launch_solver_end)
+void launch_residual(const Launch &L, int c, int mode, int par, int end);
 void launch_solver_end(const Launch &L, int last_check, int par);
 void launch_field(const Launch &L);
 
@@ -84,8 +86,8 @@ void launch_far_place(const Launch &L);
 void launch_far_rank(const Launch &L);
 void launch_scan_a(const Launch &L);
 void launch_scan_c(const Launch &L);
-void launch_far_scatter(const Launch &L);
-void launch_finish(const Launch &L, int advance_t);
+// the far list to its slots; the run's last block ends the step
+void launch_far_scatter(const Launch &L, int advance_t);
 void launch_init_finish(const Launch &L);
 
 // host sequences and state shared between translation units

@@ -175,31 +175,6 @@ __device__ __forceinline__ int src_count(const CUPIC_GLOBAL int *cnt, int ncx, i
     return cnt[(size_t)(8 - sl) * ncells + jj * ncx + ii];
 }
 
-// Exclusive scan of one value per thread over the block; returns the
-// exclusive prefix, total in *total.  Wave scans, then the wave totals.
-__device__ __forceinline__ int block_excl_scan(int v, int *sh, int &total)
-{
-    int lane = threadIdx.x & (kWave - 1), w = threadIdx.x >> 6;
-    int x = v;
-#pragma unroll
-    for (int o = 1; o < kWave; o <<= 1) {
-        int u = __shfl_up(x, o, kWave);
-        if (lane >= o) x += u;
-    }
-    if (lane == kWave - 1) sh[w] = x;
-    __syncthreads();
-    int before = 0, tot = 0;
-#pragma unroll
-    for (int q = 0; q < kBlock / kWave; q++) {
-        int t = sh[q];
-        if (q < w) before += t;
-        tot += t;
-    }
-    __syncthreads();
-    total = tot;
-    return before + x - v;
-}
-
 // Block-wide sum of one value per thread (thread 0 gets it).
 __device__ __forceinline__ int block_sum(int v, int *sh)
 {
@@ -214,9 +189,11 @@ __device__ __forceinline__ int block_sum(int v, int *sh)
     return t;
 }
 
-// Per-cell totals of the new state (near arrivals plus the far list) and
-// per-block sums.  One thread per cell (kScanCells per block, interleaved by
-// kBlock so that every access is coalesced).
+// Per-block sums of the new state's cell totals (near arrivals plus the far
+// list).  One thread per cell (kScanCells per block, interleaved by kBlock
+// so that every access is coalesced).
+constexpr int kScanPer = kScanCells / kBlock;  // cells per thread
+
 __global__ __launch_bounds__(kBlock) void k_scan_a(const RunDev *rds, const int *start,
                                                    int nruns)
 {
@@ -228,20 +205,25 @@ __global__ __launch_bounds__(kBlock) void k_scan_a(const RunDev *rds, const int 
     int blk = local - s * nblk;
     const CUPIC_GLOBAL int *cnt = gp(R.cnt[s]);
     const CUPIC_GLOBAL int *farc = gp(R.far_cnt[s]);
-    CUPIC_GLOBAL int *tot = gp(R.tot[s]);
-    int sum = 0;
+    // every load first, then the sums
+    int a[kScanPer][10];
 #pragma unroll
-    for (int u = 0; u < kScanCells / kBlock; u++) {
-        int k = blk * kScanCells + u * kBlock + threadIdx.x;
+    for (int u = 0; u < kScanPer; u++) {
+        const int k = blk * kScanCells + u * kBlock + threadIdx.x;
+#pragma unroll
+        for (int sl = 0; sl < 10; sl++) a[u][sl] = 0;
         if (k < ncells) {
-            int j = k / ncx, i = k - j * ncx;
-            int t = farc[k];
+            const int j = k / ncx, i = k - j * ncx;
+            a[u][9] = farc[k];
 #pragma unroll
-            for (int sl = 0; sl < 9; sl++) t += src_count(cnt, ncx, ncy, ncells, i, j, sl);
-            tot[k] = t;
-            sum += t;
+            for (int sl = 0; sl < 9; sl++) a[u][sl] = src_count(cnt, ncx, ncy, ncells, i, j, sl);
         }
     }
+    int sum = 0;
+#pragma unroll
+    for (int u = 0; u < kScanPer; u++)
+#pragma unroll
+        for (int sl = 0; sl < 10; sl++) sum += a[u][sl];
     int total = block_sum(sum, sh);
     if (threadIdx.x == 0) R.blk[s][blk] = total;
 }
@@ -250,50 +232,100 @@ __global__ __launch_bounds__(kBlock) void k_scan_a(const RunDev *rds, const int 
 // from slots before sl; slot 9 (far list) last.  Clears the far counters.
 // Each block starts from the sum of the block totals before it (at most a
 // few hundred, summed by the block itself); the last one also sets N_new.
+// Every load is issued before the first barrier: the block totals, and for
+// each of the thread's cells its far count and nine arrival counts, from
+// which the cell's total is summed again here.  The thread's kScanPer cells
+// are scanned together: one wave scan each, then one exchange of the wave
+// totals.
 __global__ __launch_bounds__(kBlock) void k_scan_c(const RunDev *rds, const int *start,
                                                    int nruns)
 {
     RUN_PROLOGUE();
     __shared__ int sh[kBlock / kWave];
+    __shared__ int wt[kScanPer][kBlock / kWave];
     const int ncells = R.ncells, ncx = R.ncx, ncy = R.ncy;
     int nblk = (ncells + kScanCells - 1) / kScanCells;
     int s = local / nblk;
     int blk = local - s * nblk;
-    int pre = 0;
-    for (int q = threadIdx.x; q < blk; q += kBlock) pre += R.blk[s][q];
-    pre = block_sum(pre, sh);
-    __syncthreads();  // sh is reused below
-    CUPIC_GLOBAL int *noff = gp(R.off[s][st->cur ^ 1]);
-    CUPIC_GLOBAL int *base = gp(R.base[s]);
     const CUPIC_GLOBAL int *cnt = gp(R.cnt[s]);
-    const CUPIC_GLOBAL int *tot = gp(R.tot[s]);
     CUPIC_GLOBAL int *farc = gp(R.far_cnt[s]);
+    // the parity as a scalar, so that the output pointer is a scalar load
+    // and no store waits for it
+    const int cur = __builtin_amdgcn_readfirstlane(gp(st)->cur);
+    CUPIC_GLOBAL int *noff = gp(cur ? R.off[s][0] : R.off[s][1]);
+    // block totals before this one (the first 4 kBlock in flight with the
+    // counts; more only past 4 M cells per species)
+    const CUPIC_GLOBAL int *bt = gp(R.blk[s]);
+    int v[4];
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+        const int q = m * kBlock + threadIdx.x;
+        v[m] = q < blk ? bt[q] : 0;
+    }
+    int a[kScanPer][9], t[kScanPer];
+#pragma unroll
+    for (int u = 0; u < kScanPer; u++) {
+        const int k = blk * kScanCells + u * kBlock + threadIdx.x;
+        t[u] = 0;
+#pragma unroll
+        for (int sl = 0; sl < 9; sl++) a[u][sl] = 0;
+        if (k < ncells) {
+            const int j = k / ncx, i = k - j * ncx;
+            t[u] = farc[k];
+#pragma unroll
+            for (int sl = 0; sl < 9; sl++) a[u][sl] = src_count(cnt, ncx, ncy, ncells, i, j, sl);
+        }
+    }
+    int pre = (v[0] + v[1]) + (v[2] + v[3]);
+    for (int q = 4 * kBlock + threadIdx.x; q < blk; q += kBlock) pre += bt[q];
+    pre = block_sum(pre, sh);
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x >> 6;
+    int x[kScanPer];
+#pragma unroll
+    for (int u = 0; u < kScanPer; u++) {
+#pragma unroll
+        for (int sl = 0; sl < 9; sl++) t[u] += a[u][sl];
+        x[u] = t[u];
+#pragma unroll
+        for (int o = 1; o < kWave; o <<= 1) {
+            const int y = __shfl_up(x[u], o, kWave);
+            if (lane >= o) x[u] += y;
+        }
+        if (lane == kWave - 1) wt[u][w] = x[u];
+    }
+    __syncthreads();
+    // exclusive prefix of cell (u, thread): the cells of earlier u, then
+    // the earlier waves of this u, then this wave's lanes before it
+    int base_u = pre;
+    CUPIC_GLOBAL int *base = gp(R.base[s]);
     CUPIC_GLOBAL int *farcur = gp(R.far_cur[s]);
-    int carry = pre;
-    for (int u = 0; u < kScanCells / kBlock; u++) {
-        int k = blk * kScanCells + u * kBlock + threadIdx.x;
-        bool ok = k < ncells;
-        int t = ok ? tot[k] : 0;
-        int total;
-        int ex = block_excl_scan(t, sh, total);
-        if (ok) {
-            int j = k / ncx, i = k - j * ncx;
-            int b = carry + ex;
+#pragma unroll
+    for (int u = 0; u < kScanPer; u++) {
+        int before = 0, all = 0;
+#pragma unroll
+        for (int q = 0; q < kBlock / kWave; q++) {
+            const int v = wt[u][q];
+            before += q < w ? v : 0;
+            all += v;
+        }
+        const int k = blk * kScanCells + u * kBlock + threadIdx.x;
+        if (k < ncells) {
+            int b = base_u + before + x[u] - t[u];
             noff[k] = b;
 #pragma unroll
             for (int sl = 0; sl < 9; sl++) {
                 base[(size_t)sl * ncells + k] = b;
-                b += src_count(cnt, ncx, ncy, ncells, i, j, sl);
+                b += a[u][sl];
             }
             base[(size_t)9 * ncells + k] = b;
             farc[k] = 0;
             farcur[k] = 0;
         }
-        carry += total;
+        base_u += all;
     }
     if (blk == nblk - 1 && threadIdx.x == 0) {
-        st->N_new[s] = carry;
-        noff[ncells] = carry;
+        st->N_new[s] = base_u;
+        noff[ncells] = base_u;
     }
 }
 
@@ -308,6 +340,8 @@ __global__ __launch_bounds__(kBlock) void k_far_place(const RunDev *rds, const i
     const int s = local / bps;
     if (s > 1) return;
     const int F = min(st->far_n[s] + st->n_in[s], R.far_cap);
+    const int cur = __builtin_amdgcn_readfirstlane(gp(st)->cur);
+    float4 *const out = cur ? R.part[s][0] : R.part[s][1];
     for (int e = (local - s * bps) * kBlock + threadIdx.x; e < F; e += bps * kBlock) {
         int key = R.far[s][e].key;
         int slot = atomicAdd(&R.far_cur[s][key], 1);
@@ -315,7 +349,7 @@ __global__ __launch_bounds__(kBlock) void k_far_place(const RunDev *rds, const i
         // the slot is addressed as a float4 (64-bit offset): 4 * pos as an
         // int word index would overflow past 2^29 slots
         if ((unsigned)pos < (unsigned)R.cap)
-            reinterpret_cast<int *>(R.part[s][st->cur ^ 1] + pos)[0] = e;  // first word
+            reinterpret_cast<int *>(out + pos)[0] = e;  // first word
     }
 }
 
@@ -329,12 +363,14 @@ __global__ __launch_bounds__(kBlock) void k_far_rank(const RunDev *rds, const in
     const int s = local / bps;
     if (s > 1) return;
     const int F = min(st->far_n[s] + st->n_in[s], R.far_cap);
+    const int cur = __builtin_amdgcn_readfirstlane(gp(st)->cur);
+    const int *const noff = cur ? R.off[s][0] : R.off[s][1];
+    const float4 *const slots = cur ? R.part[s][0] : R.part[s][1];
     for (int e = (local - s * bps) * kBlock + threadIdx.x; e < F; e += bps * kBlock) {
         const FarRec *far = R.far[s];
         int key = far[e].key, idx = far[e].idx;
         int b = R.base[s][9 * R.ncells + key];
-        int n = R.off[s][st->cur ^ 1][key + 1] - b;
-        const float4 *slots = R.part[s][st->cur ^ 1];
+        int n = noff[key + 1] - b;
         // four slots per round trip: their entry indices, then the entries'
         // input indices (an index outside the list reads entry 0 and counts
         // for nothing)
@@ -355,49 +391,10 @@ __global__ __launch_bounds__(kBlock) void k_far_rank(const RunDev *rds, const in
     }
 }
 
-// Far-list entries to their sorted positions, with their charge (global
-// int64 atomics; the list is short).
-__global__ __launch_bounds__(kBlock) void k_far_scatter(const RunDev *rds, const int *start,
-                                                        int nruns)
+// End of a step: flip the particle buffers, commit the counts and the probe
+// tally (probe charge accumulated in FP64: ions +1, electrons -1).
+__device__ void finish_step(const RunDev &R, RunState *st, int advance_t)
 {
-    RUN_PROLOGUE();
-    const int bps = far_blocks(R);
-    const int s = local / bps;
-    if (s > 1) return;
-    const int F = min(st->far_n[s] + st->n_in[s], R.far_cap);
-    for (int e = (local - s * bps) * kBlock + threadIdx.x; e < F; e += bps * kBlock) {
-        FarRec rec = R.far[s][e];
-        int o = st->cur ^ 1;
-        int dest = R.base[s][9 * R.ncells + rec.key] + rec.rank;
-        if ((unsigned)dest >= (unsigned)R.cap) {
-            atomicOr(&st->err, kErrSortInvariant);
-            continue;
-        }
-        st4(R.part[s][o], dest, v4f{rec.x, rec.y, rec.vx, rec.vy});
-        R.hint[s][o][dest] = 0;  // placed without a hint: pushed in full next step
-        float fx, fy;
-        int ic, jc;
-        cell_of(R, rec.x, rec.y, fx, fy, ic, jc);
-        unsigned wx, wy;
-        fixed_weights(fx, fy, wx, wy);
-        long long sgn = s == 0 ? -1 : 1;
-        long long gx = 65536 - (long long)wx, gy = 65536 - (long long)wy;
-        int i1 = ic + 1 == R.ncx ? 0 : ic + 1;
-        atomicAdd(&R.acc[jc * R.ncx + ic], (unsigned long long)(sgn * gx * gy));
-        atomicAdd(&R.acc[jc * R.ncx + i1], (unsigned long long)(sgn * (long long)wx * gy));
-        atomicAdd(&R.acc[(jc + 1) * R.ncx + ic], (unsigned long long)(sgn * gx * (long long)wy));
-        atomicAdd(&R.acc[(jc + 1) * R.ncx + i1],
-                  (unsigned long long)(sgn * (long long)wx * (long long)wy));
-    }
-}
-
-// End of cc: flip buffers, commit counts and the probe tally (probe charge
-// accumulated in FP64: ions +1, electrons -1).
-__global__ __launch_bounds__(kBlock) void k_finish(const RunDev *rds, const int *start,
-                                                   int nruns, int advance_t)
-{
-    RUN_PROLOGUE();
-    if (threadIdx.x != 0) return;
     double dq = 0.0;
     for (int s = 0; s < 2; s++) {
         st->last_probe[s] = st->probe[s];
@@ -421,6 +418,52 @@ __global__ __launch_bounds__(kBlock) void k_finish(const RunDev *rds, const int 
     st->cur ^= 1;
     st->step += 1;
     if (advance_t) st->t += R.dt;
+}
+
+// Far-list entries to their sorted positions, with their charge (global
+// int64 atomics; the list is short).  The run's last block to finish then
+// ends the step (finish_step): every block has read the state that changes
+// by then.
+__global__ __launch_bounds__(kBlock) void k_far_scatter(const RunDev *rds, const int *start,
+                                                        int nruns, int advance_t)
+{
+    RUN_PROLOGUE();
+    const int bps = far_blocks(R);
+    const int s = local / bps;  // launched with 2 bps blocks per run: s is 0 or 1
+    const int F = min(st->far_n[s] + st->n_in[s], R.far_cap);
+    const int o = __builtin_amdgcn_readfirstlane(gp(st)->cur) ^ 1;
+    for (int e = (local - s * bps) * kBlock + threadIdx.x; e < F; e += bps * kBlock) {
+        FarRec rec = R.far[s][e];
+        int dest = R.base[s][9 * R.ncells + rec.key] + rec.rank;
+        if ((unsigned)dest >= (unsigned)R.cap) {
+            atomicOr(&st->err, kErrSortInvariant);
+            continue;
+        }
+        st4(R.part[s][o], dest, v4f{rec.x, rec.y, rec.vx, rec.vy});
+        R.hint[s][o][dest] = 0;  // placed without a hint: pushed in full next step
+        float fx, fy;
+        int ic, jc;
+        cell_of(R, rec.x, rec.y, fx, fy, ic, jc);
+        unsigned wx, wy;
+        fixed_weights(fx, fy, wx, wy);
+        long long sgn = s == 0 ? -1 : 1;
+        long long gx = 65536 - (long long)wx, gy = 65536 - (long long)wy;
+        int i1 = ic + 1 == R.ncx ? 0 : ic + 1;
+        atomicAdd(&R.acc[jc * R.ncx + ic], (unsigned long long)(sgn * gx * gy));
+        atomicAdd(&R.acc[jc * R.ncx + i1], (unsigned long long)(sgn * (long long)wx * gy));
+        atomicAdd(&R.acc[(jc + 1) * R.ncx + ic], (unsigned long long)(sgn * gx * (long long)wy));
+        atomicAdd(&R.acc[(jc + 1) * R.ncx + i1],
+                  (unsigned long long)(sgn * (long long)wx * (long long)wy));
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        if (atomicAdd(&st->fin_ticket, 1u) == (unsigned)(2 * bps - 1)) {
+            __threadfence();
+            st->fin_ticket = 0u;
+            finish_step(R, st, advance_t);
+        }
+    }
 }
 
 __global__ __launch_bounds__(kBlock) void k_init_finish(const RunDev *rds, const int *start,
@@ -450,6 +493,8 @@ __global__ __launch_bounds__(kBlock) void k_init_finish(const RunDev *rds, const
     st->step = 0;
     st->t = 0.0;
     st->err = 0;
+    st->fin_ticket = 0u;
+    st->solve_ticket = 0u;
 }
 
 // ---------------------------------------------------------------- launchers
@@ -470,8 +515,7 @@ void launch_far_place(const Launch &L) { LAUNCH_K(k_far_place, L); }
 void launch_far_rank(const Launch &L) { LAUNCH_K(k_far_rank, L); }
 void launch_scan_a(const Launch &L) { LAUNCH_K(k_scan_a, L); }
 void launch_scan_c(const Launch &L) { LAUNCH_K(k_scan_c, L); }
-void launch_far_scatter(const Launch &L) { LAUNCH_K(k_far_scatter, L); }
-void launch_finish(const Launch &L, int advance_t) { LAUNCH_K(k_finish, L, advance_t); }
+void launch_far_scatter(const Launch &L, int advance_t) { LAUNCH_K(k_far_scatter, L, advance_t); }
 void launch_init_finish(const Launch &L) { LAUNCH_K(k_init_finish, L); }
 
 }  // namespace cupic

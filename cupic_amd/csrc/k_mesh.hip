@@ -28,10 +28,17 @@ __global__ __launch_bounds__(kBlock) void k_phi_linear(const RunDev *rds, const 
 // int64 node charge -> rho (volume ds^3, ds^3/2 on the two Dirichlet rows,
 // the intent of src/mesh.cu:242-268) and the Jacobi right-hand side
 // ds^2 rho / eps0 (src/mesh.cu:327).  Clears the accumulator.
+__device__ void solver_begin_work(const RunDev &R, RunState *st, int use_args, int mode,
+                                  double tol, int min_sweeps, int max_sweeps, int check_every);
+
+// begin: block 0 also starts the step's solve with the run's own settings
+// (the work of k_solver_begin), which saves a launch where a step runs the
+// conversion and the solve back to back.
 __global__ __launch_bounds__(kBlock) void k_rho_convert(const RunDev *rds, const int *start,
-                                                        int nruns)
+                                                        int nruns, int begin)
 {
     RUN_PROLOGUE();
+    if (begin && local == 0) solver_begin_work(R, st, 0, 0, 0.0, 0, 0, 0);
     int k = local * kBlock + threadIdx.x;
     if (k >= R.nnodes) return;
     int j = k / R.ncx;
@@ -45,12 +52,9 @@ __global__ __launch_bounds__(kBlock) void k_rho_convert(const RunDev *rds, const
     R.b32[k] = (float)bb;
 }
 
-__global__ __launch_bounds__(kBlock) void k_solver_begin(const RunDev *rds, const int *start,
-                                                         int nruns, int use_args, int mode,
-                                                         double tol, int min_sweeps,
-                                                         int max_sweeps, int check_every)
+__device__ void solver_begin_work(const RunDev &R, RunState *st, int use_args, int mode,
+                                  double tol, int min_sweeps, int max_sweeps, int check_every)
 {
-    RUN_PROLOGUE();
     int mx = use_args ? max_sweeps : R.step_max;
     if (mx > R.max_sweeps_alloc) mx = R.max_sweeps_alloc;
     const int md = use_args ? mode : R.step_mode;
@@ -75,6 +79,15 @@ __global__ __launch_bounds__(kBlock) void k_solver_begin(const RunDev *rds, cons
     }
 }
 
+__global__ __launch_bounds__(kBlock) void k_solver_begin(const RunDev *rds, const int *start,
+                                                         int nruns, int use_args, int mode,
+                                                         double tol, int min_sweeps,
+                                                         int max_sweeps, int check_every)
+{
+    RUN_PROLOGUE();
+    solver_begin_work(R, st, use_args, mode, tol, min_sweeps, max_sweeps, check_every);
+}
+
 // Reductions go to kRedSlots addresses per quantity (block b uses slot
 // b % kRedSlots) so that thousands of blocks do not serialise on one
 // atomic; readers take the max over the slots.
@@ -83,7 +96,8 @@ __device__ __forceinline__ double read_resid(const RunDev &R, int c)
     double m = 0.0;
 #pragma unroll
     for (int q = 0; q < kRedSlots; q++)
-        m = fmax(m, __longlong_as_double((long long)R.resid[c * kRedSlots + q]));
+        m = fmax(m, __longlong_as_double((long long)__hip_atomic_load(
+                        &R.resid[c * kRedSlots + q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)));
     return m;
 }
 
@@ -407,13 +421,11 @@ __global__ __launch_bounds__(kBlock) void k_ir_update(const RunDev *rds, const i
 // evaluates it on the FP32 phi in buffer par; mode 2 on the FP64 master and
 // also stores the correction right-hand side r = b + sum - 4 phi (mode 3
 // reads the FP64 potential too).  Each thread takes kResNodes nodes, a
-// block stride apart, so that few blocks meet at the atomic slots.
-__global__ __launch_bounds__(kBlock) void k_residual(const RunDev *rds, const int *start,
-                                                     int nruns, int c, int mode, int par)
+// block stride apart, so that few blocks meet at the atomic slots; the
+// block's maximum goes to check slot c.
+__device__ __forceinline__ void residual_block(const RunDev &R, RunState *st, double *sh,
+                                               int local, int c, int mode, int par)
 {
-    RUN_PROLOGUE();
-    __shared__ double sh[kBlock / kWave];
-    if (st->solve_done) return;
     const int ncx = R.ncx;
     const CUPIC_GLOBAL double *p64 = gp(R.phi64);
     const CUPIC_GLOBAL float *p32 = gp(R.phi32[phi_at(st->phi_cur, par)]);
@@ -446,11 +458,37 @@ __global__ __launch_bounds__(kBlock) void k_residual(const RunDev *rds, const in
     });
 }
 
-__global__ __launch_bounds__(kBlock) void k_solver_end(const RunDev *rds, const int *start,
-                                                       int nruns, int last_check, int par)
+__device__ void solver_end_work(const RunDev &R, RunState *st, int last_check, int par);
+
+// end: the run's last block to finish also ends the solve with check c (the
+// work of k_solver_end), which saves a launch after the direct solve.
+__global__ __launch_bounds__(kBlock) void k_residual(const RunDev *rds, const int *start,
+                                                     int nruns, int c, int mode, int par,
+                                                     int end)
 {
     RUN_PROLOGUE();
-    if (threadIdx.x != 0) return;
+    __shared__ double sh[kBlock / kWave];
+    const int done = st->solve_done;  // the same for the whole block
+    if (done && !end) return;
+    if (!done) residual_block(R, st, sh, local, c, mode, par);
+    if (end) {
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            __threadfence();
+            const unsigned nb = (unsigned)((R.nnodes + kBlock * kResNodes - 1) / (kBlock * kResNodes));
+            if (atomicAdd(&st->solve_ticket, 1u) == nb - 1u) {
+                __threadfence();
+                st->solve_ticket = 0u;
+                solver_end_work(R, st, c, par);
+            }
+        }
+    }
+}
+
+// End of a solve (one thread): sweep count and last residual, where the
+// solution is, and the running totals.
+__device__ void solver_end_work(const RunDev &R, RunState *st, int last_check, int par)
+{
     if (!st->solve_done) {
         int s = st->solve_mode == 3 ? 0 : st->solve_max;
         st->sweeps_done = s;
@@ -464,6 +502,13 @@ __global__ __launch_bounds__(kBlock) void k_solver_end(const RunDev *rds, const 
     if (st->solve_mode != 2) st->phi_cur = st->phi_final;
     st->sweeps_total += st->sweeps_done;
     st->solves += 1;
+}
+
+__global__ __launch_bounds__(kBlock) void k_solver_end(const RunDev *rds, const int *start,
+                                                       int nruns, int last_check, int par)
+{
+    RUN_PROLOGUE();
+    if (threadIdx.x == 0) solver_end_work(R, st, last_check, par);
 }
 
 // E = -grad phi (src/mesh.cu:413-468): central differences with periodic x
@@ -508,7 +553,7 @@ __global__ __launch_bounds__(kBlock) void k_field(const RunDev *rds, const int *
     } while (0)
 
 void launch_phi_linear(const Launch &L) { LAUNCH_K(k_phi_linear, L); }
-void launch_rho_convert(const Launch &L) { LAUNCH_K(k_rho_convert, L); }
+void launch_rho_convert(const Launch &L, int begin) { LAUNCH_K(k_rho_convert, L, begin); }
 void launch_solver_begin(const Launch &L, int use_args, int mode, double tol, int min_sweeps,
                          int max_sweeps, int check_every)
 {
@@ -536,9 +581,9 @@ void launch_jacobi_block(const Launch &L, int halo, int s0, int nsw, int mode, i
 #undef JAC_LAUNCH
 }
 void launch_ir_update(const Launch &L, int par) { LAUNCH_K(k_ir_update, L, par); }
-void launch_residual(const Launch &L, int c, int mode, int par)
+void launch_residual(const Launch &L, int c, int mode, int par, int end)
 {
-    LAUNCH_K(k_residual, L, c, mode, par);
+    LAUNCH_K(k_residual, L, c, mode, par, end);
 }
 void launch_solver_end(const Launch &L, int last_check, int par)
 {
