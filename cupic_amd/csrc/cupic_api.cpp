@@ -224,9 +224,11 @@ struct TimeScope {
 };
 
 // begun: the solve was started by the charge conversion (launch_rho_convert
-// with begin set, use_args 0), so no k_solver_begin launch.
+// with begin set, use_args 0), so no k_solver_begin launch.  field_ends: a
+// direct solve leaves its residual and its end to the field kernel that
+// follows (seq_field with resid_end set).
 void seq_solve(const Target &T, int use_args, int mode, double tol, int min_sweeps,
-               int max_sweeps, int check, bool begun = false)
+               int max_sweeps, int check, bool begun = false, bool field_ends = false)
 {
     TimeScope ts(T, kTSolve);
     if (!begun)
@@ -250,7 +252,8 @@ void seq_solve(const Target &T, int use_args, int mode, double tol, int min_swee
             launch_dft_gemm(T.L(step == 0 || step == 3 ? kShapeDftGemmX : kShapeDftGemmY), step);
             launch_dft_reduce(T.L(kShapeDftReduce), step);
         }
-        launch_residual(T.L(kShapeResid), 0, 3, 0, 1);  // its last block ends the solve
+        if (!field_ends)
+            launch_residual(T.L(kShapeResid), 0, 3, 0, 1);  // its last block ends the solve
         return;
     }
     const bool narrow = T.nblk[kShapeJacBlk] >= kJacNarrowMinBlocks;
@@ -348,16 +351,16 @@ void seq_rho(const Target &T, int begin_solve = 0)
     launch_rho_convert(T.L(kShapeNodes), begin_solve);
 }
 
-void seq_field(const Target &T)
+void seq_field(const Target &T, int resid_end = 0)
 {
     TimeScope ts(T, kTField);
-    launch_field(T.L(kShapeNodes));
+    launch_field(T.L(kShapeNodes), resid_end);
 }
 
-void seq_mover(const Target &T)
+void seq_mover(const Target &T, int end_solve = 0)
 {
     TimeScope ts(T, kTPushRank);
-    launch_push_rank(T.L(kShapeCountTasks));
+    launch_push_rank(T.L(kShapeCountTasks), end_solve);
 }
 
 void seq_cc(const Target &T, int use_host_t, double t, int advance_t, bool clock_forked)
@@ -527,6 +530,22 @@ int charge_deposition(cupic_run *r, int begin_solve)
     seq_rho(T, begin_solve);
     r->acc_valid = false;
     return check_launch(r, "charge_deposition");
+}
+
+// particle_mover; end_solve: the counting pass also ends the step's direct
+// solve (after seq_field with resid_end set)
+int particle_mover(cupic_run *r, int end_solve)
+{
+    if (!r) return set_err(nullptr, CUPIC_ERR_ARG, "null run");
+    if (!r->initialised) return set_err(&r->err, CUPIC_ERR_STATE, "run not initialised");
+    if (r->moved) return set_err(&r->err, CUPIC_ERR_STATE, "particle_mover called twice");
+    if (r->acc_valid) {  // an unconsumed deposit would be added to
+        HIPTRY(&r->err, hipMemsetAsync(r->h.acc, 0, (size_t)r->h.nnodes * 8, r->stream));
+        r->acc_valid = false;
+    }
+    seq_mover(target_of(r), end_solve);
+    r->moved = 1;
+    return check_launch(r, "particle_mover");
 }
 
 }  // namespace
@@ -979,19 +998,7 @@ int cupic_field_solver(cupic_run *r)
 
 // ---- particles and boundary conditions
 
-int cupic_particle_mover(cupic_run *r)
-{
-    if (!r) return set_err(nullptr, CUPIC_ERR_ARG, "null run");
-    if (!r->initialised) return set_err(&r->err, CUPIC_ERR_STATE, "run not initialised");
-    if (r->moved) return set_err(&r->err, CUPIC_ERR_STATE, "particle_mover called twice");
-    if (r->acc_valid) {  // an unconsumed deposit would be added to
-        HIPTRY(&r->err, hipMemsetAsync(r->h.acc, 0, (size_t)r->h.nnodes * 8, r->stream));
-        r->acc_valid = false;
-    }
-    seq_mover(target_of(r));
-    r->moved = 1;
-    return check_launch(r, "particle_mover");
-}
+int cupic_particle_mover(cupic_run *r) { return particle_mover(r, 0); }
 
 int cupic_cc(cupic_run *r, double t)
 {
@@ -1015,12 +1022,13 @@ int cupic_step(cupic_run *r, double t)
         r->clock_forked = true;
     }
     if (!st) {
+        const bool direct = r->p.jacobi_mode == 3;
         seq_solve(target_of(r), 0, r->p.jacobi_mode, 0, 0, r->p.jacobi_max_sweeps,
-                  r->p.jacobi_check_every, true);
-        seq_field(target_of(r));
+                  r->p.jacobi_check_every, true, direct);
+        seq_field(target_of(r), direct);
         st = check_launch(r, "solve/field");
+        if (!st) st = particle_mover(r, direct);
     }
-    if (!st) st = cupic_particle_mover(r);
     if (!st) st = cupic_cc(r, t);
     return st;
 }
@@ -1542,9 +1550,9 @@ void batch_step_sequence(cupic_batch *b, const Target &T)
 {
     launch_rho_convert(T.L(kShapeNodes), 1);
     seq_clock_fork(T, 0, 0.0);
-    seq_solve(T, 0, b->mode, 0, 0, b->max_sweeps, b->check, true);
-    launch_field(T.L(kShapeNodes));
-    launch_push_rank(T.L(kShapeCountTasks));
+    seq_solve(T, 0, b->mode, 0, 0, b->max_sweeps, b->check, true, b->mode == 3);
+    launch_field(T.L(kShapeNodes), b->mode == 3);
+    launch_push_rank(T.L(kShapeCountTasks), b->mode == 3);
     seq_cc(T, 0, 0.0, 1, true);
 }
 

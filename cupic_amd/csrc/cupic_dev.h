@@ -268,6 +268,46 @@ __host__ __device__ inline int phi_alt(int a) { return a == 0 ? 1 : 0; }
 __host__ __device__ inline int phi_at(int a, int par) { return (par & 1) ? phi_alt(a) : a; }
 __host__ __device__ inline int phi_third(int a) { return 3 - a - phi_alt(a); }
 
+// Reductions go to kRedSlots addresses per quantity (block b uses slot
+// b % kRedSlots) so that thousands of blocks do not serialise on one
+// atomic; readers take the max over the slots.
+__device__ __forceinline__ double read_resid(const RunDev &R, int c)
+{
+    double m = 0.0;
+#pragma unroll
+    for (int q = 0; q < kRedSlots; q++)
+        m = fmax(m, __longlong_as_double((long long)__hip_atomic_load(
+                        &R.resid[c * kRedSlots + q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)));
+    return m;
+}
+
+__device__ __forceinline__ float read_sweep_max(const RunDev &R, int s)
+{
+    float m = 0.0f;
+#pragma unroll
+    for (int q = 0; q < kRedSlots; q++) m = fmaxf(m, __uint_as_float(R.sweep_max[s * kRedSlots + q]));
+    return m;
+}
+
+// End of a solve (one thread): sweep count and last residual, where the
+// solution is, and the running totals.
+__device__ inline void solver_end_work(const RunDev &R, RunState *st, int last_check, int par)
+{
+    if (!st->solve_done) {
+        int s = st->solve_mode == 3 ? 0 : st->solve_max;
+        st->sweeps_done = s;
+        if (st->solve_mode == 0)
+            st->last_resid = s > 0 ? (double)read_sweep_max(R, s - 1) : 0.0;
+        else
+            st->last_resid = read_resid(R, last_check);
+        st->phi_final = phi_at(st->phi_cur, par);
+        st->solve_done = 1;
+    }
+    if (st->solve_mode != 2) st->phi_cur = st->phi_final;
+    st->sweeps_total += st->sweeps_done;
+    st->solves += 1;
+}
+
 // Node charge a (int64, 2^-32 of a macro-particle charge) of row j -> rho
 // (volume ds^3, ds^3/2 on the two Dirichlet rows, the intent of
 // src/mesh.cu:242-268) and the Jacobi right-hand side ds^2 rho / eps0

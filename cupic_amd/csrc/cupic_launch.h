@@ -60,7 +60,10 @@ void launch_ir_update(const Launch &L, int par);
 // end: the last block also ends the solve at check c (as launch_solver_end)
 void launch_residual(const Launch &L, int c, int mode, int par, int end);
 void launch_solver_end(const Launch &L, int last_check, int par);
-void launch_field(const Launch &L);
+// resid_end (direct solve only, instead of its launch_residual): the field
+// kernel also reduces the solve's residual; launch_push_rank with end_solve
+// set must follow
+void launch_field(const Launch &L, int resid_end);
 
 // direct solve (mode 3)
 void launch_dft_init(const Launch &L);
@@ -75,7 +78,9 @@ void launch_create(const Launch &L);
 void launch_init_sort(const Launch &L);
 void launch_deposit(const Launch &L);
 void launch_half_kick(const Launch &L);
-void launch_push_rank(const Launch &L);
+// end_solve: block 0 also ends a direct solve whose residual the field
+// kernel reduced (launch_field with resid_end set)
+void launch_push_rank(const Launch &L, int end_solve);
 // seg_stayers: the instance for fewer than kSegStayersPpc particles per cell
 void launch_scatter(const Launch &L, bool seg_stayers);
 

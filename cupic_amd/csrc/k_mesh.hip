@@ -88,26 +88,7 @@ __global__ __launch_bounds__(kBlock) void k_solver_begin(const RunDev *rds, cons
     solver_begin_work(R, st, use_args, mode, tol, min_sweeps, max_sweeps, check_every);
 }
 
-// Reductions go to kRedSlots addresses per quantity (block b uses slot
-// b % kRedSlots) so that thousands of blocks do not serialise on one
-// atomic; readers take the max over the slots.
-__device__ __forceinline__ double read_resid(const RunDev &R, int c)
-{
-    double m = 0.0;
-#pragma unroll
-    for (int q = 0; q < kRedSlots; q++)
-        m = fmax(m, __longlong_as_double((long long)__hip_atomic_load(
-                        &R.resid[c * kRedSlots + q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)));
-    return m;
-}
 
-__device__ __forceinline__ float read_sweep_max(const RunDev &R, int s)
-{
-    float m = 0.0f;
-#pragma unroll
-    for (int q = 0; q < kRedSlots; q++) m = fmaxf(m, __uint_as_float(R.sweep_max[s * kRedSlots + q]));
-    return m;
-}
 
 // Ends a solve at sweep s with residual res; the solution is in phi buffer
 // phi_at(phi_cur, par), or in the third buffer when par < 0 (a check inside
@@ -458,8 +439,6 @@ __device__ __forceinline__ void residual_block(const RunDev &R, RunState *st, do
     });
 }
 
-__device__ void solver_end_work(const RunDev &R, RunState *st, int last_check, int par);
-
 // end: the run's last block to finish also ends the solve with check c (the
 // work of k_solver_end), which saves a launch after the direct solve.
 __global__ __launch_bounds__(kBlock) void k_residual(const RunDev *rds, const int *start,
@@ -485,24 +464,6 @@ __global__ __launch_bounds__(kBlock) void k_residual(const RunDev *rds, const in
     }
 }
 
-// End of a solve (one thread): sweep count and last residual, where the
-// solution is, and the running totals.
-__device__ void solver_end_work(const RunDev &R, RunState *st, int last_check, int par)
-{
-    if (!st->solve_done) {
-        int s = st->solve_mode == 3 ? 0 : st->solve_max;
-        st->sweeps_done = s;
-        if (st->solve_mode == 0)
-            st->last_resid = s > 0 ? (double)read_sweep_max(R, s - 1) : 0.0;
-        else
-            st->last_resid = read_resid(R, last_check);
-        st->phi_final = phi_at(st->phi_cur, par);
-        st->solve_done = 1;
-    }
-    if (st->solve_mode != 2) st->phi_cur = st->phi_final;
-    st->sweeps_total += st->sweeps_done;
-    st->solves += 1;
-}
 
 __global__ __launch_bounds__(kBlock) void k_solver_end(const RunDev *rds, const int *start,
                                                        int nruns, int last_check, int par)
@@ -514,33 +475,51 @@ __global__ __launch_bounds__(kBlock) void k_solver_end(const RunDev *rds, const 
 // E = -grad phi (src/mesh.cu:413-468): central differences with periodic x
 // on interior rows; one-sided Ey and Ex = 0 on rows 0 and ncy.  Evaluated in
 // FP64 from the FP32 (modes 0/1) or FP64 (modes 2/3) potential.
+// kResid (a step's direct solve, launched instead of k_residual): the FP64
+// residual of the solve, which reads the same neighbours, is reduced into
+// check slot 0 here too; the next kernel ends the solve (solver_end_work).
+template <bool kResid>
 __global__ __launch_bounds__(kBlock) void k_field(const RunDev *rds, const int *start,
                                                   int nruns)
 {
     RUN_PROLOGUE();
-    int k = local * kBlock + threadIdx.x;
-    if (k >= R.nnodes) return;
-    int ncx = R.ncx;
-    int j = k / ncx;
-    int i = k - j * ncx;
-    bool m2 = (st->solve_mode >= 2);
-    const float *p32 = R.phi32[st->phi_cur];
-    const double *p64 = R.phi64;
-    auto P = [&](int q) -> double { return m2 ? p64[q] : (double)p32[q]; };
-    double ex, ey;
-    if (j == 0) {
-        ex = 0.0;
-        ey = (P(k) - P(k + ncx)) / R.ds;
-    } else if (j == R.ncy) {
-        ex = 0.0;
-        ey = (P(k - ncx) - P(k)) / R.ds;
-    } else {
-        int kw = (i == 0) ? k + ncx - 1 : k - 1;
-        int ke = (i == ncx - 1) ? k - ncx + 1 : k + 1;
-        ex = (P(kw) - P(ke)) / (2.0 * R.ds);
-        ey = (P(k - ncx) - P(k + ncx)) / (2.0 * R.ds);
+    const int k = local * kBlock + threadIdx.x;
+    double rmax = 0.0;
+    if (k < R.nnodes) {
+        const int ncx = R.ncx;
+        const int j = k / ncx;
+        const int i = k - j * ncx;
+        const bool m2 = kResid || (st->solve_mode >= 2);
+        const float *p32 = R.phi32[st->phi_cur];
+        const double *p64 = R.phi64;
+        auto P = [&](int q) -> double { return m2 ? p64[q] : (double)p32[q]; };
+        double ex, ey;
+        if (j == 0) {
+            ex = 0.0;
+            ey = (P(k) - P(k + ncx)) / R.ds;
+        } else if (j == R.ncy) {
+            ex = 0.0;
+            ey = (P(k - ncx) - P(k)) / R.ds;
+        } else {
+            const int kw = (i == 0) ? k + ncx - 1 : k - 1;
+            const int ke = (i == ncx - 1) ? k - ncx + 1 : k + 1;
+            const double pw = P(kw), pe = P(ke), pn = P(k + ncx), ps = P(k - ncx);
+            ex = (pw - pe) / (2.0 * R.ds);
+            ey = (ps - pn) / (2.0 * R.ds);
+            if (kResid) {
+                const double sum = (((R.b64[k] + pw) + pe) + pn) + ps;
+                rmax = fabs(0.25 * (sum - 4.0 * P(k)));
+            }
+        }
+        R.E[k] = make_float2((float)ex, (float)ey);
     }
-    R.E[k] = make_float2((float)ex, (float)ey);
+    if (kResid) {
+        __shared__ double sh[kBlock / kWave];
+        unsigned long long *slot = &R.resid[local & (kRedSlots - 1)];
+        block_max_atomic(rmax, sh, [&](double m) {
+            if (m > 0.0) atomicMax(slot, (unsigned long long)__double_as_longlong(m));
+        });
+    }
 }
 
 // ---------------------------------------------------------------- launchers
@@ -589,6 +568,12 @@ void launch_solver_end(const Launch &L, int last_check, int par)
 {
     LAUNCH_K(k_solver_end, L, last_check, par);
 }
-void launch_field(const Launch &L) { LAUNCH_K(k_field, L); }
+void launch_field(const Launch &L, int resid_end)
+{
+    if (resid_end)
+        LAUNCH_K(k_field<true>, L);
+    else
+        LAUNCH_K(k_field<false>, L);
+}
 
 }  // namespace cupic

@@ -925,10 +925,13 @@ __device__ __forceinline__ void scatter_task(const RunDev &R, RunState *st, cons
     if (__ballot(bad) && lane == 0) atomicOr(&st->err, kErrSortInvariant);
 }
 
+// end_solve: block 0 first ends the step's direct solve, whose residual the
+// field kernel reduced (solver_end_work), which saves a launch.
 __global__ __launch_bounds__(kBlock) void k_push_rank(const RunDev *rds, const int *start,
-                                                      int nruns)
+                                                      int nruns, int end_solve)
 {
     RUN_PROLOGUE();
+    if (end_solve && local == 0 && threadIdx.x == 0) solver_end_work(R, st, 0, 0);
     __shared__ int cnts[kBlock / kWave][kCountCells * 9];
     __shared__ int gone[kBlock / kWave][kCountCells];
     __shared__ int bnds[kBlock / kWave][kCountCells + 1];
@@ -969,7 +972,7 @@ void launch_create(const Launch &L) { LAUNCH_K(k_create, L); }
 void launch_init_sort(const Launch &L) { LAUNCH_K(k_init_sort, L); }
 void launch_deposit(const Launch &L) { LAUNCH_K(k_deposit, L); }
 void launch_half_kick(const Launch &L) { LAUNCH_K(k_half_kick, L); }
-void launch_push_rank(const Launch &L) { LAUNCH_K(k_push_rank, L); }
+void launch_push_rank(const Launch &L, int end_solve) { LAUNCH_K(k_push_rank, L, end_solve); }
 void launch_scatter(const Launch &L, bool seg_stayers)
 {
     if (seg_stayers)
