@@ -61,7 +61,7 @@ struct cupic_run {
     int nblk[kShapeCount] = {};
     hipStream_t stream = nullptr;
     hipStream_t side = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_field = nullptr;
     std::vector<void *> allocs;
     std::string err;
     int moved = 0;          // particle_mover done, cc pending
@@ -86,7 +86,7 @@ struct cupic_batch {
     int nblk[kShapeCount] = {};
     hipStream_t stream = nullptr;
     hipStream_t side = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_field = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     bool use_graph = false;
@@ -112,6 +112,7 @@ int shape_blocks(const RunDev &h, Shape s)
     case kShapeNodes: return (h.nnodes + kBlock - 1) / kBlock;
     case kShapeResid: return (h.nnodes + kBlock * kResNodes - 1) / (kBlock * kResNodes);
     case kShapeTasks: return (2 * h.ncy * h.tpr + 3) / 4;
+    case kShapeScatter: return 2 * far_blocks(h) + (2 * h.ncy * h.tpr + 3) / 4;
     case kShapeCountTasks: return (2 * h.ncy * h.tpr1 + 3) / 4;
     case kShapeCellBlk: return 2 * ((h.ncells + kScanCells - 1) / kScanCells);
     case kShapeRunSp: return 2;
@@ -166,7 +167,7 @@ struct Target {
     hipStream_t stream;
     cupic_run *timed;     // non-null: record timing events for this run
     hipStream_t side = nullptr;   // injection clock runs here when set
-    hipEvent_t fork = nullptr, join = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr, field = nullptr;
     bool seg_stayers = false;     // scatter instance for low particles per cell
     Launch L(Shape s) const
     {
@@ -184,16 +185,28 @@ Target target_of(cupic_run *r)
     T.side = r->side;
     T.fork = r->ev_fork;
     T.join = r->ev_join;
+    T.field = r->ev_field;
     T.seg_stayers = (int64_t)r->p.per_row < (int64_t)kSegStayersPpc * r->p.ncx;
     return T;
 }
 
 // Starts the injection clock of the coming cc on the side stream.
+// A step's injection on the side stream: the clock at the step's start
+// (seq_clock_fork), the injection once the field is ready (seq_inject_fork,
+// it reads E), beside the counting pass, whose part of the far list and
+// whose counts it does not touch; seq_cc with clock_forked set waits for it.
 void seq_clock_fork(const Target &T, int use_host_t, double t)
 {
     (void)hipEventRecord(T.fork, T.stream);
     (void)hipStreamWaitEvent(T.side, T.fork, 0);
     launch_clock(T.Lside(kShapeRun), use_host_t, t);
+}
+
+void seq_inject_fork(const Target &T)
+{
+    (void)hipEventRecord(T.field, T.stream);
+    (void)hipStreamWaitEvent(T.side, T.field, 0);
+    launch_inject(T.Lside(kShapeInject));
     (void)hipEventRecord(T.join, T.side);
 }
 
@@ -367,25 +380,25 @@ void seq_cc(const Target &T, int use_host_t, double t, int advance_t, bool clock
 {
     {
         TimeScope ts(T, kTInject);
-        if (clock_forked)
+        if (clock_forked) {  // clock and injection ran on the side stream
             (void)hipStreamWaitEvent(T.stream, T.join, 0);
-        else
+        } else {
             launch_clock(T.L(kShapeRun), use_host_t, t);
-        launch_inject(T.L(kShapeInject));
+            launch_inject(T.L(kShapeInject));
+        }
     }
     {
         TimeScope ts(T, kTScan);
         launch_scan_a(T.L(kShapeCellBlk));
         launch_scan_c(T.L(kShapeCellBlk));
-        launch_far_place(T.L(kShapeFar));
-        launch_far_rank(T.L(kShapeFar));
     }
     {
         TimeScope ts(T, kTScatter);
-        launch_scatter(T.L(kShapeTasks), T.seg_stayers);
+        launch_scatter(T.L(kShapeScatter), T.seg_stayers);  // also places the far list
     }
     {
         TimeScope ts(T, kTFinish);
+        launch_far_rank(T.L(kShapeFar));
         launch_far_scatter(T.L(kShapeFar), advance_t);
     }
 }
@@ -872,6 +885,7 @@ int cupic_run_create(const cupic_params *pin, int64_t capacity_hint, cupic_run *
         hipStreamCreateWithFlags(&r->side, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&r->ev_fork, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&r->ev_join, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&r->ev_field, hipEventDisableTiming) != hipSuccess ||
         hipMemcpy(r->d_rd, &h, sizeof(RunDev), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(r->d_plan, plan.data(), plan.size() * sizeof(int), hipMemcpyHostToDevice) !=
             hipSuccess ||
@@ -900,6 +914,7 @@ int cupic_run_destroy(cupic_run *r)
     for (void *p : r->allocs) (void)hipFree(p);
     if (r->ev_fork) (void)hipEventDestroy(r->ev_fork);
     if (r->ev_join) (void)hipEventDestroy(r->ev_join);
+    if (r->ev_field) (void)hipEventDestroy(r->ev_field);
     if (r->side) (void)hipStreamDestroy(r->side);
     if (r->stream) (void)hipStreamDestroy(r->stream);
     delete r;
@@ -1026,6 +1041,7 @@ int cupic_step(cupic_run *r, double t)
         seq_solve(target_of(r), 0, r->p.jacobi_mode, 0, 0, r->p.jacobi_max_sweeps,
                   r->p.jacobi_check_every, true, direct);
         seq_field(target_of(r), direct);
+        seq_inject_fork(target_of(r));
         st = check_launch(r, "solve/field");
         if (!st) st = particle_mover(r, direct);
     }
@@ -1534,6 +1550,7 @@ Target batch_target(cupic_batch *b)
     T.side = b->side;
     T.fork = b->ev_fork;
     T.join = b->ev_join;
+    T.field = b->ev_field;
     // one scatter launch serves every run: the instance follows the
     // particles per cell over the whole batch
     int64_t part = 0, cells = 0;
@@ -1552,6 +1569,7 @@ void batch_step_sequence(cupic_batch *b, const Target &T)
     seq_clock_fork(T, 0, 0.0);
     seq_solve(T, 0, b->mode, 0, 0, b->max_sweeps, b->check, true, b->mode == 3);
     launch_field(T.L(kShapeNodes), b->mode == 3);
+    seq_inject_fork(T);
     launch_push_rank(T.L(kShapeCountTasks), b->mode == 3);
     seq_cc(T, 0, 0.0, 1, true);
 }
@@ -1633,6 +1651,7 @@ int cupic_batch_create(cupic_run *const *runs, int32_t nruns, int32_t use_graph,
     BATCH_TRY(hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking));
     BATCH_TRY(hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming));
     BATCH_TRY(hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming));
+    BATCH_TRY(hipEventCreateWithFlags(&b->ev_field, hipEventDisableTiming));
     BATCH_TRY(hipDeviceSynchronize());
     if (b->use_graph) {
         Target T = batch_target(b);
@@ -1688,6 +1707,7 @@ int cupic_batch_destroy(cupic_batch *b)
     if (b->side) (void)hipStreamSynchronize(b->side);
     if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
     if (b->ev_join) (void)hipEventDestroy(b->ev_join);
+    if (b->ev_field) (void)hipEventDestroy(b->ev_field);
     if (b->side) (void)hipStreamDestroy(b->side);
     if (b->d_rds) (void)hipFree(b->d_rds);
     if (b->d_plan) (void)hipFree(b->d_plan);

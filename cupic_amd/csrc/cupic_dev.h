@@ -94,7 +94,7 @@ struct RunState {
     int N[2];                       // particles per species (current buffer)
     int N_new[2];
     int n_in[2];                    // injected this step
-    int far_n[2];                   // fast movers in the far list (injected follow)
+    int far_n[2];                   // fast movers in the far list (see far_entry)
     int tin_started;
     int step;
     double t;
@@ -205,6 +205,24 @@ __host__ __device__ inline int far_blocks(const RunDev &h)
 {
     int b = (h.far_cap + kBlock - 1) / kBlock;
     return b < kFarBlocks ? b : kFarBlocks;
+}
+
+// A species' far list holds the fast movers of the counting pass from slot
+// 0 (at most far_mov_cap of them) and the injected particles from slot
+// far_mov_cap (at most max_inject), so that the two can be written
+// concurrently.  Its entries in order, movers then injected, are numbered
+// v = 0 .. far_len - 1.
+__host__ __device__ inline int far_mov_cap(const RunDev &h) { return h.far_cap - h.max_inject; }
+
+__device__ inline int far_len(const RunDev &R, const RunState *st, int s)
+{
+    return min(st->far_n[s], far_mov_cap(R)) + st->n_in[s];
+}
+
+__device__ inline int far_entry(const RunDev &R, const RunState *st, int s, int v)
+{
+    const int fm = min(st->far_n[s], far_mov_cap(R));
+    return v < fm ? v : far_mov_cap(R) + (v - fm);
 }
 
 // ---------------------------------------------------------------- helpers

@@ -24,6 +24,7 @@ enum Shape : int {
     kShapeNodes = 0,   // ceil(nnodes / 256)
     kShapeResid,       // ceil(nnodes / (256 kResNodes))
     kShapeTasks,       // ceil(2 * ncy * tpr / 4): one wave per (species, row, column block)
+    kShapeScatter,     // 2 far_blocks() + the kShapeTasks blocks: far-list places, then tasks
     kShapeCountTasks,  // ceil(2 * ncy * tpr1 / 4): the same for the counting pass
     kShapeCellBlk,     // 2 * ceil(ncells / kScanCells)
     kShapeRunSp,       // 2: one block per species
@@ -82,12 +83,11 @@ void launch_half_kick(const Launch &L);
 // kernel reduced (launch_field with resid_end set)
 void launch_push_rank(const Launch &L, int end_solve);
 // seg_stayers: the instance for fewer than kSegStayersPpc particles per cell
-void launch_scatter(const Launch &L, bool seg_stayers);
+void launch_scatter(const Launch &L, bool seg_stayers);  // kShapeScatter
 
 // boundary conditions and sort
 void launch_clock(const Launch &L, int use_host_t, double t_host);
 void launch_inject(const Launch &L);
-void launch_far_place(const Launch &L);
 void launch_far_rank(const Launch &L);
 void launch_scan_a(const Launch &L);
 void launch_scan_c(const Launch &L);

@@ -151,12 +151,7 @@ __global__ __launch_bounds__(kBlock) void k_inject(const RunDev *rds, const int 
     rec.idx = st->N[s] + k;
     rec.rank = 0;
     rec.pad = 0;
-    int slot = st->far_n[s] + k;  // after the fast movers of pass 1
-    if (slot >= R.far_cap) {
-        atomicOr(&st->err, kErrFarOverflow);
-        return;
-    }
-    R.far[s][slot] = rec;
+    R.far[s][far_mov_cap(R) + k] = rec;  // k < n_in <= max_inject
     atomicAdd(&R.far_cnt[s][rec.key], 1);
 }
 
@@ -329,30 +324,6 @@ __global__ __launch_bounds__(kBlock) void k_scan_c(const RunDev *rds, const int 
     }
 }
 
-// Far list, after the scan: each entry takes a place in its cell's far
-// region (any order) and writes its list index into the first word of that
-// slot of the output buffer, which the far scatter overwrites last.
-__global__ __launch_bounds__(kBlock) void k_far_place(const RunDev *rds, const int *start,
-                                                      int nruns)
-{
-    RUN_PROLOGUE();
-    const int bps = far_blocks(R);
-    const int s = local / bps;
-    if (s > 1) return;
-    const int F = min(st->far_n[s] + st->n_in[s], R.far_cap);
-    const int cur = __builtin_amdgcn_readfirstlane(gp(st)->cur);
-    float4 *const out = cur ? R.part[s][0] : R.part[s][1];
-    for (int e = (local - s * bps) * kBlock + threadIdx.x; e < F; e += bps * kBlock) {
-        int key = R.far[s][e].key;
-        int slot = atomicAdd(&R.far_cur[s][key], 1);
-        int pos = R.base[s][9 * R.ncells + key] + slot;
-        // the slot is addressed as a float4 (64-bit offset): 4 * pos as an
-        // int word index would overflow past 2^29 slots
-        if ((unsigned)pos < (unsigned)R.cap)
-            reinterpret_cast<int *>(out + pos)[0] = e;  // first word
-    }
-}
-
 // Stable rank of a far entry within its cell: entries of the same cell with
 // a smaller input index (a handful per cell).
 __global__ __launch_bounds__(kBlock) void k_far_rank(const RunDev *rds, const int *start,
@@ -362,11 +333,12 @@ __global__ __launch_bounds__(kBlock) void k_far_rank(const RunDev *rds, const in
     const int bps = far_blocks(R);
     const int s = local / bps;
     if (s > 1) return;
-    const int F = min(st->far_n[s] + st->n_in[s], R.far_cap);
+    const int F = far_len(R, st, s);
     const int cur = __builtin_amdgcn_readfirstlane(gp(st)->cur);
     const int *const noff = cur ? R.off[s][0] : R.off[s][1];
     const float4 *const slots = cur ? R.part[s][0] : R.part[s][1];
-    for (int e = (local - s * bps) * kBlock + threadIdx.x; e < F; e += bps * kBlock) {
+    for (int v = (local - s * bps) * kBlock + threadIdx.x; v < F; v += bps * kBlock) {
+        const int e = far_entry(R, st, s, v);
         const FarRec *far = R.far[s];
         int key = far[e].key, idx = far[e].idx;
         int b = R.base[s][9 * R.ncells + key];
@@ -382,7 +354,7 @@ __global__ __launch_bounds__(kBlock) void k_far_rank(const RunDev *rds, const in
                 q[k] = u + k < n ? reinterpret_cast<const int *>(slots + (b + u + k))[0] : -1;
 #pragma unroll
             for (int k = 0; k < 4; k++) {
-                const bool ok = (unsigned)q[k] < (unsigned)F;
+                const bool ok = (unsigned)q[k] < (unsigned)R.far_cap;
                 const int other = far[ok ? q[k] : 0].idx;
                 rank += (ok && other < idx) ? 1 : 0;
             }
@@ -430,10 +402,10 @@ __global__ __launch_bounds__(kBlock) void k_far_scatter(const RunDev *rds, const
     RUN_PROLOGUE();
     const int bps = far_blocks(R);
     const int s = local / bps;  // launched with 2 bps blocks per run: s is 0 or 1
-    const int F = min(st->far_n[s] + st->n_in[s], R.far_cap);
+    const int F = far_len(R, st, s);
     const int o = __builtin_amdgcn_readfirstlane(gp(st)->cur) ^ 1;
-    for (int e = (local - s * bps) * kBlock + threadIdx.x; e < F; e += bps * kBlock) {
-        FarRec rec = R.far[s][e];
+    for (int v = (local - s * bps) * kBlock + threadIdx.x; v < F; v += bps * kBlock) {
+        FarRec rec = R.far[s][far_entry(R, st, s, v)];
         int dest = R.base[s][9 * R.ncells + rec.key] + rec.rank;
         if ((unsigned)dest >= (unsigned)R.cap) {
             atomicOr(&st->err, kErrSortInvariant);
@@ -511,7 +483,6 @@ void launch_clock(const Launch &L, int use_host_t, double t_host)
     LAUNCH_K(k_clock, L, use_host_t, t_host);
 }
 void launch_inject(const Launch &L) { LAUNCH_K(k_inject, L); }
-void launch_far_place(const Launch &L) { LAUNCH_K(k_far_place, L); }
 void launch_far_rank(const Launch &L) { LAUNCH_K(k_far_rank, L); }
 void launch_scan_a(const Launch &L) { LAUNCH_K(k_scan_a, L); }
 void launch_scan_c(const Launch &L) { LAUNCH_K(k_scan_c, L); }

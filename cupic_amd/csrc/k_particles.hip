@@ -456,7 +456,7 @@ __device__ __forceinline__ void count_task(const RunDev &R, RunState *st, const 
 {
     const int s = T.s, j0 = T.j0, col0 = T.col0, nc = T.nc;
     const Geo G = geo_of(R);
-    const int far_cap = R.far_cap;
+    const int far_cap = far_mov_cap(R);  // fast movers' part of the far list
     int *const far_cnt = R.far_cnt[s];
     FarRec *const farl = R.far[s];
     const float qdtm = R.sp[s].qdtm;
@@ -945,6 +945,31 @@ __global__ __launch_bounds__(kBlock) void k_push_rank(const RunDev *rds, const i
                threadIdx.x & (kWave - 1));
 }
 
+// Far list, after the scan: each entry takes a place in its cell's far
+// region (any order) and writes its list index into the first word of that
+// slot of the output buffer, which the far scatter overwrites last.  Blocks
+// 0 .. 2 far_blocks - 1 of the scatter launch (far_blocks per species): the
+// work depends on the scan alone, as the scatter's does, and the scatter
+// does not touch far slots.
+__device__ __forceinline__ void far_place(const RunDev &R, RunState *st, int local)
+{
+    const int bps = far_blocks(R);
+    const int s = local / bps;
+    const int F = far_len(R, st, s);
+    const int cur = __builtin_amdgcn_readfirstlane(gp(st)->cur);
+    float4 *const out = cur ? R.part[s][0] : R.part[s][1];
+    for (int v = (local - s * bps) * kBlock + threadIdx.x; v < F; v += bps * kBlock) {
+        const int e = far_entry(R, st, s, v);
+        int key = R.far[s][e].key;
+        int slot = atomicAdd(&R.far_cur[s][key], 1);
+        int pos = R.base[s][9 * R.ncells + key] + slot;
+        // the slot is addressed as a float4 (64-bit offset): 4 * pos as an
+        // int word index would overflow past 2^29 slots
+        if ((unsigned)pos < (unsigned)R.cap)
+            reinterpret_cast<int *>(out + pos)[0] = e;  // first word
+    }
+}
+
 template <bool kSegStayers>
 __global__ __launch_bounds__(kBlock) void k_scatter(const RunDev *rds, const int *start,
                                                     int nruns)
@@ -953,7 +978,12 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const RunDev *rds, const int
     __shared__ int cnts[kBlock / kWave][kCmax * 9];
     __shared__ float2 etiles[kBlock / kWave][2 * (kCmax + 1)];
     __shared__ unsigned long long tiles[kBlock / kWave][4 * kTileW];
-    TaskInfo T = task_of(R, local, R.C, R.tpr);
+    const int nfar = 2 * far_blocks(R);
+    if (local < nfar) {
+        far_place(R, st, local);
+        return;
+    }
+    TaskInfo T = task_of(R, local - nfar, R.C, R.tpr);
     if (!T.ok) return;
     int w = threadIdx.x >> 6;
     scatter_task<kSegStayers>(R, st, T, cnts[w], etiles[w], tiles[w], threadIdx.x & (kWave - 1));
