@@ -199,12 +199,13 @@ struct RunDev {
 };
 
 // Blocks per species of the far-list kernels: grid-stride loops over the
-// entries, so a run with a large capacity and a short list launches few
-// idle blocks.
+// entries, one thread per entry of a usual list (max_inject is twice a
+// step's injection plus a margin; fast movers are rare), at most
+// kFarBlocks, so that a batch of many small runs launches few idle blocks.
 __host__ __device__ inline int far_blocks(const RunDev &h)
 {
-    int b = (h.far_cap + kBlock - 1) / kBlock;
-    return b < kFarBlocks ? b : kFarBlocks;
+    const int b = (h.max_inject + kBlock - 1) / kBlock;
+    return b < 1 ? 1 : (b < kFarBlocks ? b : kFarBlocks);
 }
 
 // A species' far list holds the fast movers of the counting pass from slot

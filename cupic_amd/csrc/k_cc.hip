@@ -393,9 +393,10 @@ __device__ void finish_step(const RunDev &R, RunState *st, int advance_t)
 }
 
 // Far-list entries to their sorted positions, with their charge (global
-// int64 atomics; the list is short).  The run's last block to finish then
-// ends the step (finish_step): every block has read the state that changes
-// by then.
+// int64 atomics; the list is short).  The run's last block to take a ticket
+// then ends the step (finish_step).  No fence is needed: the step's end
+// reads only state that earlier kernels wrote, and every block has used the
+// state it changes (the list length, the parity) before taking its ticket.
 __global__ __launch_bounds__(kBlock) void k_far_scatter(const RunDev *rds, const int *start,
                                                         int nruns, int advance_t)
 {
@@ -428,13 +429,9 @@ __global__ __launch_bounds__(kBlock) void k_far_scatter(const RunDev *rds, const
                   (unsigned long long)(sgn * (long long)wx * (long long)wy));
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        if (atomicAdd(&st->fin_ticket, 1u) == (unsigned)(2 * bps - 1)) {
-            __threadfence();
-            st->fin_ticket = 0u;
-            finish_step(R, st, advance_t);
-        }
+    if (threadIdx.x == 0 && atomicAdd(&st->fin_ticket, 1u) == (unsigned)(2 * bps - 1)) {
+        st->fin_ticket = 0u;
+        finish_step(R, st, advance_t);
     }
 }
 
