@@ -654,20 +654,21 @@ def main():
         solve_ms = out["kernels_ms_per_launch"]["solve"]
         m, nx = p.ncy - 1, p.ncx
         if direct_uses_fft(p.ncx, p.ncy):
-            # the FFT path: three transform kernels and the FP64 residual;
-            # algorithmic bytes: b and the Dirichlet rows in (8 B per node),
-            # the x spectra out and in (2 x 8), the y solve's result out and
-            # in (2 x 8), the FP64 and FP32 potential out (12), the residual
-            # reading b and the potential (16): 68 B per interior node
-            nbytes = 68.0 * m * nx
+            # the FFT path in a step: the three transform kernels (the solve
+            # is started by the charge conversion, its FP64 residual is
+            # reduced by the field kernel); algorithmic bytes: b and the
+            # Dirichlet rows in (8 B per node), the x spectra out and in
+            # (2 x 8), the y solve's result out and in (2 x 8), the FP64 and
+            # FP32 potential out (12): 52 B per interior node
+            nbytes = 52.0 * m * nx
             ach = nbytes / (solve_ms * 1e-3) if solve_ms > 0 else 0.0
             out["solve_roofline"] = {"bound": "hbm", "achieved": ach / 1e9,
                                      "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                                      "frac": ach / HBM_PEAK, "bytes_per_solve": nbytes,
                                      "avg_launch_ms": solve_ms,
-                                     "note": "whole solve (set-up, three FFT kernels, FP64 "
-                                             "residual, end): six dependent launches of a "
-                                             "few microseconds each, launch-latency bound"}
+                                     "note": "the solve's three FFT kernels: three dependent "
+                                             "launches of about 12 us each, launch-latency "
+                                             "bound"}
         else:
             # the dense path's four FP64 products: 2 x (m x ncx x ncx) for the
             # DFT along x and 2 x (m x m x ncx) for the sine transform along
@@ -677,8 +678,8 @@ def main():
             out["solve_roofline"] = {"bound": "mfma", "achieved": ach, "peak": FP64_MFMA_PEAK,
                                      "unit": "TFLOP/s", "frac": ach / FP64_MFMA_PEAK,
                                      "flops_per_solve": flops, "avg_launch_ms": solve_ms,
-                                     "note": "whole solve (four products, slice sums, FP64 "
-                                             "residual) against the dense FP64 matrix peak"}
+                                     "note": "the solve's four products and slice sums "
+                                             "against the dense FP64 matrix peak"}
     # measured HBM bytes of a whole step (every kernel, rocprofv3 counters,
     # copy-calibrated) over the measured step time
     tstep, src = profiled_traffic("step") if a.config == "prod" else (None, None)
