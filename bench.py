@@ -234,6 +234,10 @@ def direct_uses_fft(ncx, ncy):
     return ncy >= 2 and ok(ncx) and ok(2 * ncy)
 
 
+# direct_method of the batched and served runs (--direct-method)
+DIRECT_METHOD = 0
+
+
 def batched_leg(cp, n_runs, steps, seq_steps, seed, fixed_tile=0, solver="direct"):
     """One HIP-graph step advances all runs; baseline: the same runs stepped
     one after another through the plain path (cupic_step per run), skipped
@@ -247,7 +251,8 @@ def batched_leg(cp, n_runs, steps, seq_steps, seed, fixed_tile=0, solver="direct
     mode = cp.POISSON_DIRECT if solver == "direct" else cp.JACOBI_RESIDUAL
     for d in draws:
         p = cp.params(**d, jacobi_mode=mode, jacobi_tol=1e-6,
-                      jacobi_tol_init=1e-6, jacobi_max_sweeps=500, jacobi_check_every=50)
+                      jacobi_tol_init=1e-6, jacobi_max_sweeps=500, jacobi_check_every=50,
+                      direct_method=DIRECT_METHOD)
         r = cp.Run(p)
         r.init_sim()
         runs.append(r)
@@ -311,6 +316,7 @@ def batched_leg(cp, n_runs, steps, seq_steps, seed, fixed_tile=0, solver="direct
     rs_s = n_runs * seq_steps / wall_s if seq_steps > 0 else None
     return {"runs": n_runs, "particles": int(npart), "steps": steps,
             "tile_cells": tile, "tile_tuning_ms_per_step": tune,
+            "direct_method": ("auto", "dense", "fft")[DIRECT_METHOD],
             "ms_per_batched_step": 1e3 * wall_b / steps,
             "run_steps_per_s": rs_b, "particle_steps_per_s": npart * steps / wall_b,
             "sequential_plain_run_steps_per_s": rs_s,
@@ -336,7 +342,8 @@ def draw_requests(n, rate, seed):
         t += float(rng.exponential(1.0 / rate))
         out.append((t, dict(ncx=int(rng.integers(128, 1025)), ncy=int(rng.integers(64, 513)),
                             ppc=int(rng.integers(20, 201)),
-                            phi_p=float(rng.uniform(-10.0, 2.0)), run_id=1000 + k)))
+                            phi_p=float(rng.uniform(-10.0, 2.0)), run_id=1000 + k,
+                            direct_method=DIRECT_METHOD)))
     return out
 
 
@@ -466,6 +473,7 @@ def serving_leg(cp, n_req, rate, max_steps, seed, plain="full"):
                        f"Te/Ti 1; converged when the net probe current changes < {rel:g} of "
                        f"the gross current between {window}-step windows (cap {max_steps} "
                        f"steps); direct FP64 Poisson solve",
+           "direct_method": ("auto", "dense", "fft")[DIRECT_METHOD],
            "served": served, "plain_one_at_a_time": plain_res,
            "points_per_s_vs_plain": served["points_per_s"] / plain_res["points_per_s"]}
     if plain == "full":
@@ -500,6 +508,10 @@ def main():
                          "sequential baseline from --batched-seq-steps) and print its JSON")
     ap.add_argument("--batched-seq-steps", type=int, default=50,
                     help="timed plain steps of every run for the sequential baseline")
+    ap.add_argument("--direct-method", default="auto", choices=["auto", "dense", "fft"],
+                    help="direct Poisson solve of the batched and served runs: FFT for "
+                         "power-of-two lengths else dense products (auto), dense products "
+                         "always, or FFT for any length (Bluestein)")
     ap.add_argument("--serve-requests", type=int, default=50)
     ap.add_argument("--serve-rate", type=float, default=50.0)
     ap.add_argument("--serve-max-steps", type=int, default=10000)
@@ -518,6 +530,8 @@ def main():
               "(multi-GPU sharding is a design note, not part of this build)", file=sys.stderr)
         sys.exit(2)
     solver = a.solver or ("direct" if a.config == "prod" else "jacobi")
+    global DIRECT_METHOD
+    DIRECT_METHOD = {"auto": 0, "dense": 1, "fft": 2}[a.direct_method]
 
     import cupic_amd as cp
     name, cus = cp.init_dev(0)

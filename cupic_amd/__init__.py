@@ -23,6 +23,7 @@ ERRORS = {-1: "ARG", -2: "HIP", -3: "ALLOC", -4: "CAPACITY", -5: "STATE", -6: "N
           -7: "IO"}
 
 JACOBI_REFERENCE, JACOBI_RESIDUAL, JACOBI_REFINE, POISSON_DIRECT = 0, 1, 2, 3
+DIRECT_AUTO, DIRECT_DENSE, DIRECT_FFT = 0, 1, 2
 HINTS_OFF, HINTS_ON, HINTS_CHECKED = 0, 1, 2
 QUIRK_Q7_REFERENCE_KICK = 1
 QUIRK_Q10_SKIP_FIRST_LINE = 2
@@ -47,7 +48,7 @@ class Params(C.Structure):
         ("quirks", C.c_uint32),
         ("jacobi_mode", C.c_int32), ("jacobi_min_sweeps", C.c_int32),
         ("jacobi_max_sweeps", C.c_int32), ("jacobi_check_every", C.c_int32),
-        ("tile_cells", C.c_int32),
+        ("tile_cells", C.c_int32), ("direct_method", C.c_int32),
         ("jacobi_tol", C.c_double), ("jacobi_tol_init", C.c_double),
         ("Lx", C.c_double), ("Ly", C.c_double),
         ("dtin_e", C.c_double), ("dtin_i", C.c_double),

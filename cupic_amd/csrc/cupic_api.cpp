@@ -99,6 +99,15 @@ namespace cupic {
 // Direct solve: output tiles of one product over the m = ncy - 1 interior
 // rows and ncx columns, and the k slices that bring a product near
 // kGSlices workgroups (never more slices than k tiles).
+// The direct solve's method for a run: its FFT instance (fft_class), or 0
+// for the dense products.
+int direct_class(const cupic_params &p)
+{
+    return p.direct_method == CUPIC_DIRECT_DENSE
+               ? 0
+               : fft_class(p.ncx, p.ncy, p.direct_method == CUPIC_DIRECT_FFT);
+}
+
 int dft_tiles(int ncx, int m) { return ((m + kGM - 1) / kGM) * ((ncx + kGN - 1) / kGN); }
 int dft_slices(int ncx, int m, int Kd)
 {
@@ -130,8 +139,10 @@ int shape_blocks(const RunDev &h, Shape s)
     case kShapeDftInit: {
         if (!h.dft_d) return 0;
         const int64_t m = h.ncy - 1;
-        if (h.dft_fft)  // twiddles of both lengths, then the eigenvalue parts
-            return (int)((2 * (int64_t)h.ncx + 2 * (int64_t)h.ncy + m + kBlock - 1) / kBlock);
+        if (h.dft_fft) {  // twiddles and chirps of both lengths, then the eigenvalue parts
+            const FftTabs t = fft_tabs(h.ncx, h.ncy);
+            return (int)(((int64_t)t.bh_x + h.ncx + m + kBlock - 1) / kBlock);
+        }
         return (int)(((int64_t)h.ncx * h.ncx + kBlock - 1) / kBlock + (m * m + kBlock - 1) / kBlock +
                      (h.ncx + m + kBlock - 1) / kBlock);
     }
@@ -145,9 +156,14 @@ int shape_blocks(const RunDev &h, Shape s)
         if (!h.dft_f) return 0;
         return (int)(((int64_t)(h.ncy - 1) * h.ncx + kBlock - 1) / kBlock);
     case kShapeFftRowsS: return h.dft_fft == 1 ? h.ncy / 2 : 0;  // ceil(m / 2), m = ncy - 1
-    case kShapeFftColsS: return h.dft_fft == 1 ? h.ncx / 2 : 0;
-    case kShapeFftRowsL: return h.dft_fft == 2 ? h.ncy / 2 : 0;
-    case kShapeFftColsL: return h.dft_fft == 2 ? h.ncx / 2 : 0;
+    case kShapeFftColsS: return h.dft_fft == 1 ? (h.ncx + 1) / 2 : 0;
+    case kShapeFftRowsM: return h.dft_fft == 2 ? h.ncy / 2 : 0;
+    case kShapeFftColsM: return h.dft_fft == 2 ? (h.ncx + 1) / 2 : 0;
+    case kShapeFftRowsL: return h.dft_fft == 3 ? h.ncy / 2 : 0;
+    case kShapeFftColsL: return h.dft_fft == 3 ? (h.ncx + 1) / 2 : 0;
+    case kShapeFftTabS: return h.dft_fft == 1 ? 2 : 0;
+    case kShapeFftTabM: return h.dft_fft == 2 ? 2 : 0;
+    case kShapeFftTabL: return h.dft_fft == 3 ? 2 : 0;
     default: return 0;
     }
 }
@@ -256,11 +272,10 @@ void seq_solve(const Target &T, int use_args, int mode, double tol, int min_swee
         // direct FP64 solve of the same system, then its FP64 residual: by
         // FFT where the lengths allow it, else by dense products (in a
         // batch each run takes its own; launches with no blocks are skipped)
-        for (int step = 0; step < 3; step++) {
-            const bool cols = step == 1;
-            launch_fft(T.L(cols ? kShapeFftColsS : kShapeFftRowsS), step, false);
-            launch_fft(T.L(cols ? kShapeFftColsL : kShapeFftRowsL), step, true);
-        }
+        static const Shape rows[3] = {kShapeFftRowsS, kShapeFftRowsM, kShapeFftRowsL};
+        static const Shape cols[3] = {kShapeFftColsS, kShapeFftColsM, kShapeFftColsL};
+        for (int step = 0; step < 3; step++)
+            for (int c = 0; c < 3; c++) launch_fft(T.L(step == 1 ? cols[c] : rows[c]), step, c + 1);
         for (int step = 0; step < 4; step++) {
             launch_dft_gemm(T.L(step == 0 || step == 3 ? kShapeDftGemmX : kShapeDftGemmY), step);
             launch_dft_reduce(T.L(kShapeDftReduce), step);
@@ -473,6 +488,9 @@ int validate(const cupic_params *p, std::string *err)
         return set_err(err, CUPIC_ERR_ARG, "jacobi_mode must be 0, 1, 2 or 3");
     if (p->tile_cells < 0 || p->tile_cells > kCmax)
         return set_err(err, CUPIC_ERR_ARG, "tile_cells must be 0 (automatic) or 1..16");
+    if (p->direct_method < CUPIC_DIRECT_AUTO || p->direct_method > CUPIC_DIRECT_FFT)
+        return set_err(err, CUPIC_ERR_ARG,
+                       "direct_method must be 0 (automatic), 1 (dense) or 2 (FFT)");
     if (p->jacobi_max_sweeps < 1 || p->jacobi_max_sweeps > 1000000)
         return set_err(err, CUPIC_ERR_ARG, "jacobi_max_sweeps out of range");
     double ncells = (double)p->ncx * p->ncy;
@@ -749,8 +767,8 @@ int cupic_run_footprint(const cupic_params *pin, int64_t capacity_hint, int64_t 
     b += (int64_t)(p.jacobi_max_sweeps + 2) * 12 * kRedSlots;
     if (p.jacobi_mode == 3) {
         const int64_t nx = p.ncx, m = p.ncy - 1;
-        if (fft_class(p.ncx, p.ncy)) {
-            b += 8 * (2 * (nx + 2 * (int64_t)p.ncy) + nx + m + 2 * m * nx);
+        if (direct_class(p)) {
+            b += 16 * (int64_t)fft_tabs(p.ncx, p.ncy).total + 8 * (nx + m + 2 * m * nx);
         } else {
             const int64_t ns = std::max(dft_slices(p.ncx, p.ncy - 1, p.ncx),
                                         dft_slices(p.ncx, p.ncy - 1, p.ncy - 1));
@@ -853,9 +871,9 @@ int cupic_run_create(const cupic_params *pin, int64_t capacity_hint, cupic_run *
     h.resid = dev_alloc<unsigned long long>(r, ((size_t)p.jacobi_max_sweeps + 2) * kRedSlots, &st);
     if (p.jacobi_mode == 3) {
         const size_t nx = (size_t)p.ncx, m = (size_t)p.ncy - 1;
-        h.dft_fft = fft_class(p.ncx, p.ncy);
+        h.dft_fft = direct_class(p);
         if (h.dft_fft) {
-            h.dft_tw = dev_alloc<double>(r, 2 * (nx + 2 * (size_t)p.ncy), &st);
+            h.dft_tw = dev_alloc<double>(r, 2 * (size_t)fft_tabs(p.ncx, p.ncy).total, &st);
         } else {
             h.nsx = dft_slices(p.ncx, p.ncy - 1, p.ncx);
             h.nsy = dft_slices(p.ncx, p.ncy - 1, p.ncy - 1);
@@ -895,6 +913,8 @@ int cupic_run_create(const cupic_params *pin, int64_t capacity_hint, cupic_run *
     }
     if (h.dft_d) {
         launch_dft_init(target_of(r).L(kShapeDftInit));
+        static const Shape tabs[3] = {kShapeFftTabS, kShapeFftTabM, kShapeFftTabL};
+        for (int c = 0; c < 3; c++) launch_fft(target_of(r).L(tabs[c]), 3, c + 1);
         if (hipStreamSynchronize(r->stream) != hipSuccess) {
             cupic_run_destroy(r);
             return set_err(nullptr, CUPIC_ERR_HIP, "direct-solve tables failed");

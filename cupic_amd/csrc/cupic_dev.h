@@ -45,18 +45,59 @@ constexpr int kGN = 64;             // direct solve: GEMM output tile columns (4
 constexpr int kGK = 32;             // direct solve: GEMM k tile
 constexpr int kGSlices = 512;       // direct solve: workgroups a product aims for (k slices)
 constexpr int kGMaxSlices = 2;   // at most: each slice adds a partial grid to write and sum
-// Direct solve by FFT: transform lengths (ncx along x, 2 ncy along y) that
-// are powers of two up to kFftL; two kernel instances, for lengths up to
-// kFftS and up to kFftL (LDS: two arrays of the length in complex doubles).
+// Direct solve by FFT: a transform of length n (ncx along x, 2 ncy along y)
+// runs as one FFT of length n when n is a power of two, else (where the
+// run's direct_method allows it) by Bluestein's chirp z-transform as two
+// FFTs of the power of two P >= 2 n - 1.  Three kernel instances, for FFT
+// lengths up to kFftS, kFftM and kFftL (LDS: two arrays of the length in
+// complex doubles).
 constexpr int kFftS = 1024;
+constexpr int kFftM = 2048;
 constexpr int kFftL = 4096;
-__host__ __device__ constexpr bool fft_len_ok(int n) { return n >= 2 && n <= kFftL && (n & (n - 1)) == 0; }
-// 0: the dense products; 1: FFT, lengths up to kFftS; 2: FFT, up to kFftL
-__host__ __device__ constexpr int fft_class(int ncx, int ncy)
+__host__ __device__ constexpr bool pow2_len(int n) { return n >= 1 && (n & (n - 1)) == 0; }
+// the FFT length that transforms n points (n itself, or Bluestein's P)
+__host__ __device__ constexpr int fft_len(int n)
 {
-    return !(ncy >= 2 && fft_len_ok(ncx) && fft_len_ok(2 * ncy))
-               ? 0
-               : (ncx <= kFftS && 2 * ncy <= kFftS ? 1 : 2);
+    if (pow2_len(n)) return n;
+    int p = 1;
+    while (p < 2 * n - 1) p *= 2;
+    return p;
+}
+// 0: the dense products; 1, 2, 3: FFT, lengths up to kFftS, kFftM, kFftL.
+// any_len: Bluestein's transform for lengths that are not powers of two
+// (else those grids take the dense products).
+__host__ __device__ constexpr int fft_class(int ncx, int ncy, bool any_len)
+{
+    const bool fits = ncx >= 2 && ncx <= kFftL && ncy >= 2 && ncy <= kFftL &&
+                      (any_len || (pow2_len(ncx) && pow2_len(2 * ncy)));
+    const int lx = fits ? fft_len(ncx) : 2 * kFftL;
+    const int ly = fits ? fft_len(2 * ncy) : 2 * kFftL;
+    const int l = lx > ly ? lx : ly;
+    return l > kFftL ? 0 : (l <= kFftS ? 1 : (l <= kFftM ? 2 : 3));
+}
+// The FFT tables of a run, in complex doubles (re, im) from dft_tw:
+// twiddles e^(-2 pi i t / Lx) (Lx = fft_len(ncx)) and e^(-2 pi i t / Ly)
+// (Ly = fft_len(2 ncy)), the chirps w_n = e^(-i pi n^2 / N) for N = ncx and
+// N = 2 ncy, and Bluestein's filter spectra FFT(conj w, wrapped) / Lx and
+// / Ly (used only where the length is not a power of two).
+struct FftTabs {
+    int lx, ly;
+    int tw_x, tw_y, ch_x, ch_y, bh_x, bh_y;  // offsets in complex doubles
+    int total;
+};
+__host__ __device__ inline FftTabs fft_tabs(int ncx, int ncy)
+{
+    FftTabs t;
+    t.lx = fft_len(ncx);
+    t.ly = fft_len(2 * ncy);
+    t.tw_x = 0;
+    t.tw_y = t.lx;
+    t.ch_x = t.tw_y + t.ly;
+    t.ch_y = t.ch_x + ncx;
+    t.bh_x = t.ch_y + 2 * ncy;
+    t.bh_y = t.bh_x + t.lx;
+    t.total = t.bh_y + t.ly;
+    return t;
 }
 constexpr int kJacRows = 8;         // blocked Jacobi: rows per wave
 constexpr int kJacWaves = 8;        // blocked Jacobi: waves per workgroup
@@ -192,9 +233,8 @@ struct RunDev {
     double *dft_d;                  // mode 3: [ncx + m] eigenvalue parts per column, per row
     double *dft_z, *dft_y;          // mode 3: [m][ncx] transformed rows (two stages)
     double *dft_p;                  // mode 3: [max(nsx, nsy)][m][ncx] partial sums
-    int dft_fft;                    // mode 3: fft_class(ncx, ncy) (0: the dense products)
-    double *dft_tw;                 // mode 3, FFT: twiddles e^(-2 pi i k / L), L = ncx then
-                                    // 2 ncy (re, im pairs)
+    int dft_fft;                    // mode 3: the FFT instance, fft_class (0: the dense products)
+    double *dft_tw;                 // mode 3, FFT: the tables of fft_tabs (re, im pairs)
     RunState *st;
 };
 

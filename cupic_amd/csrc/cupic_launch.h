@@ -35,14 +35,20 @@ enum Shape : int {
     kShapeInit,        // 2 * ceil(n_init / 256)
     kShapeJacBlk,      // ceil(ncx / jac_tx(kJacH)) * ceil((ncy - 1) / jac_ty(kJacH))
     kShapeJacBlkN,     // the same for the narrow halo kJacHNarrow
-    kShapeDftInit,     // ceil(ncx^2 / 256) + ceil(m^2 / 256) + ceil((ncx + m) / 256), m = ncy - 1
+    kShapeDftInit,     // dense: ceil(ncx^2 / 256) + ceil(m^2 / 256) + ceil((ncx + m) / 256),
+                       // m = ncy - 1; FFT: ceil((twiddles + chirps + ncx + m) / 256)
     kShapeDftGemmX,    // ceil(m / kGM) * ceil(ncx / kGN) * nsx
     kShapeDftGemmY,    // ceil(m / kGM) * ceil(ncx / kGN) * nsy
     kShapeDftReduce,   // ceil(m * ncx / 256)
     kShapeFftRowsS,    // ceil(m / 2) when fft_class is 1: row pairs
-    kShapeFftColsS,    // ncx / 2 when fft_class is 1: column pairs
-    kShapeFftRowsL,    // the same for fft_class 2
+    kShapeFftColsS,    // ceil(ncx / 2) when fft_class is 1: column pairs
+    kShapeFftTabS,     // 2 when fft_class is 1: Bluestein filter spectra (x, y)
+    kShapeFftRowsM,    // the same three for fft_class 2
+    kShapeFftColsM,
+    kShapeFftTabM,
+    kShapeFftRowsL,    // and for fft_class 3
     kShapeFftColsL,
+    kShapeFftTabL,
     kShapeCount
 };
 
@@ -71,8 +77,9 @@ void launch_dft_init(const Launch &L);
 void launch_dft_gemm(const Launch &L, int step);
 void launch_dft_reduce(const Launch &L, int step);
 // step 0: forward x transform of row pairs; 1: the y solve of column pairs;
-// 2: inverse x transform into the potential.  large: the kFftL instance.
-void launch_fft(const Launch &L, int step, bool large);
+// 2: inverse x transform into the potential; 3: Bluestein's filter spectra
+// (set-up, after launch_dft_init).  cls: the instance, fft_class 1, 2 or 3.
+void launch_fft(const Launch &L, int step, int cls);
 
 // particles
 void launch_create(const Launch &L);

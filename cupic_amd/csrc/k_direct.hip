@@ -3,8 +3,10 @@
 // The Jacobi iteration of src/mesh.cu:297-381 converges to the solution of
 // the 5-point system 4 phi - (sum of the four neighbours) = b on rows
 // 1..ncy-1, periodic in x, with the Dirichlet rows 0 (probe) and ncy.  This
-// mode computes that fixed point directly, as four FP64 matrix products on
-// the matrix cores (m = ncy - 1 interior rows):
+// mode computes that fixed point directly, by fast transforms where their
+// lengths fit the FFT kernels (the FFT path below, any ncx and ncy whose
+// FFT lengths are at most kFftL), else as four FP64 matrix products on the
+// matrix cores (m = ncy - 1 interior rows):
 //
 //   1. R^ = R F: a real DFT along x of the right-hand side rows, with the
 //      Dirichlet rows moved into the first and last of them (the known
@@ -141,18 +143,27 @@ __global__ __launch_bounds__(kBlock) void k_dft_init(const RunDev *rds, const in
     RUN_PROLOGUE();
     const int ncx = R.ncx, m = R.ncy - 1;
     if (R.dft_fft) {
-        // twiddles e^(-2 pi i t / L) for L = ncx, then L = 2 ncy (t / L exact),
-        // then the eigenvalue parts as below
-        const int L2 = 2 * R.ncy;
+        // twiddles e^(-2 pi i t / L) for L = Lx, then L = Ly (t / L exact),
+        // the chirps w_n = e^(-i pi n^2 / N) for N = ncx, then N = 2 ncy
+        // (n^2 reduced exactly mod 2N), then the eigenvalue parts as below;
+        // Bluestein's filter spectra follow from the chirps (k_fft step 3)
+        const FftTabs T = fft_tabs(ncx, R.ncy);
         int e = local * kBlock + threadIdx.x;
         double2 *tw = reinterpret_cast<double2 *>(R.dft_tw);
-        if (e < ncx + L2) {
-            const int L = e < ncx ? ncx : L2, t = e < ncx ? e : e - ncx;
+        if (e < T.ch_x) {
+            const int L = e < T.tw_y ? T.lx : T.ly, t = e < T.tw_y ? e : e - T.tw_y;
             const double a = 2.0 * (double)t / (double)L;
             tw[e] = make_double2(cospi(a), -sinpi(a));
             return;
         }
-        e -= ncx + L2;
+        if (e < T.bh_x) {
+            const int N = e < T.ch_y ? ncx : 2 * R.ncy;
+            const long long n = e < T.ch_y ? e - T.ch_x : e - T.ch_y;
+            const double a = (double)((n * n) % (2LL * N)) / (double)N;
+            tw[e] = make_double2(cospi(a), -sinpi(a));
+            return;
+        }
+        e -= T.bh_x;
         if (e < ncx) {
             const double sk = sinpi((double)dft_k_of(e, ncx) / (double)ncx);
             R.dft_d[e] = 4.0 * sk * sk;
@@ -272,38 +283,58 @@ __global__ __launch_bounds__(kBlock) void k_dft_reduce(const RunDev *rds, const 
 
 // ---------------------------------------------------------------- FFT path
 //
-// With ncx and 2 ncy powers of two (up to kFftL) the same solve runs as fast
+// Where both transforms fit (fft_class), the same solve runs as fast
 // transforms, three kernels of one workgroup per row or column pair:
 //   0. rows r, r+1 of R (the right-hand side with the Dirichlet rows added,
 //      as for the dense product) as one complex row z = R_r + i R_{r+1}; a
-//      forward FFT of length ncx split into the two real spectra
+//      forward DFT of length ncx split into the two real spectra
 //      X1 = (Z_k + conj Z_{-k}) / 2, X2 = (Z_k - conj Z_{-k}) / 2i, stored as
 //      the dense path's ncx real coefficients per row, column-major
 //      (dft_z[c][r]): Re X_k in the cos columns, Im X_k in the sin columns;
-//   1. columns c, c+1 as one complex column x + i y: the sine transform of
-//      length m as an FFT of length 2(m+1) of the odd extension
+//   1. columns c, c+1 as one complex column x + i y (c+1 absent for the
+//      last column of an odd ncx): the sine transform of length m as a DFT
+//      of length 2(m+1) of the odd extension
 //      (0, v_0 .. v_{m-1}, 0, -v_{m-1} .. -v_0), whose coefficient l+1 is
 //      -2i X_l + 2 Y_l; scaled by sqrt(2/(m+1)) and divided by mu(l, k),
 //      then transformed again the same way (S is its own inverse), into
 //      dft_y[c][r];
 //   2. rows r, r+1: both real spectra as one complex spectrum extended by
-//      Hermitian symmetry, transformed back (conjugate, forward FFT,
+//      Hermitian symmetry, transformed back (conjugate, forward DFT,
 //      conjugate, / ncx) into the two rows of the potential.
+// A DFT of power-of-two length is one FFT.  Any other length N runs as
+// Bluestein's chirp z-transform: with w_n = e^(-i pi n^2 / N),
+// nk = (n^2 + k^2 - (k-n)^2) / 2 gives X_k = w_k sum_n (x_n w_n) conj w_(k-n),
+// a cyclic convolution of length P >= 2N - 1 (P a power of two): the
+// chirped input, zero-padded, is transformed, multiplied by the filter
+// spectrum FFT(b) / P (b_j = conj w_j and b_(P-j) = conj w_j, set up once per
+// run), and transformed back (conjugate, FFT, conjugate).
 // The FFT: iterative Stockham in LDS (natural order in and out), radix-4
 // stages and, when log2 L is odd, a last radix-2 stage; each stage reads
 // its butterflies into registers, synchronises and writes in place.  The
-// twiddles e^(-2 pi i t / L) come from the run's table (set up once).
+// twiddles e^(-2 pi i t / L) come from the run's table.
 
 __device__ __forceinline__ double2 cmul(double2 a, double2 b)
 {
     return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
+// Per-thread unroll bounds of an instance (one transform per workgroup):
+// radix-4 butterflies per thread and stage, points per thread, and sine
+// coefficients per thread (m < P / 2).
+template <int LMAX>
+struct FftDims {
+    static constexpr int kB4 = LMAX / (4 * kBlock) > 1 ? LMAX / (4 * kBlock) : 1;
+    static constexpr int kE = 4 * kB4;
+    static constexpr int kU = 2 * kB4;
+};
+
+// FFT of length L (a power of two) of buf[0 .. L) in place, by the
+// workgroup.
 template <int LMAX>
 __device__ void fft_lds(double2 *buf, const double2 *tw, int L)
 {
-    constexpr int kB4 = (LMAX / 4 + kBlock - 1) / kBlock;  // radix-4 butterflies per thread
-    constexpr int kB2 = (LMAX / 2 + kBlock - 1) / kBlock;
+    const int T = kBlock, lt = threadIdx.x;
+    constexpr int kB4 = FftDims<LMAX>::kB4, kB2 = 2 * kB4;
     const int lg = 31 - __clz(L);
     int Ns = 1;
     for (int stage = 0; stage < lg / 2; stage++) {
@@ -311,7 +342,7 @@ __device__ void fft_lds(double2 *buf, const double2 *tw, int L)
         double2 o[kB4][4];
 #pragma unroll
         for (int u = 0; u < kB4; u++) {
-            const int j = threadIdx.x + u * kBlock;
+            const int j = lt + u * T;
             if (j < nb) {
                 const int k = j & (Ns - 1);
                 const double2 v0 = buf[j];
@@ -331,7 +362,7 @@ __device__ void fft_lds(double2 *buf, const double2 *tw, int L)
         __syncthreads();
 #pragma unroll
         for (int u = 0; u < kB4; u++) {
-            const int j = threadIdx.x + u * kBlock;
+            const int j = lt + u * T;
             if (j < nb) {
                 const int k = j & (Ns - 1);
                 const int d = (j - k) * 4 + k;
@@ -347,7 +378,7 @@ __device__ void fft_lds(double2 *buf, const double2 *tw, int L)
         double2 o[kB2][2];
 #pragma unroll
         for (int u = 0; u < kB2; u++) {
-            const int j = threadIdx.x + u * kBlock;
+            const int j = lt + u * T;
             if (j < nb) {
                 const int k = j & (Ns - 1);
                 const double2 v0 = buf[j];
@@ -359,7 +390,7 @@ __device__ void fft_lds(double2 *buf, const double2 *tw, int L)
         __syncthreads();
 #pragma unroll
         for (int u = 0; u < kB2; u++) {
-            const int j = threadIdx.x + u * kBlock;
+            const int j = lt + u * T;
             if (j < nb) {
                 const int k = j & (Ns - 1);
                 const int d = (j - k) * 2 + k;
@@ -371,28 +402,114 @@ __device__ void fft_lds(double2 *buf, const double2 *tw, int L)
     }
 }
 
-// The global loads of a kernel are all issued before the first LDS store
-// (kE elements per thread in registers), so that they cost one round trip.
+typedef double v2d __attribute__((ext_vector_type(2)));
 
-// step 0: row pair g -> the two rows' real spectra, column-major in dft_z
+// Complex entry i of a run's FFT table section at offset off (fft_tabs),
+// as a global load.
+struct CTab {
+    const double *p;
+    __device__ __forceinline__ double2 operator[](int i) const
+    {
+        const v2d v = ((const CUPIC_GLOBAL v2d *)p)[i];
+        return make_double2(v.x, v.y);
+    }
+};
+__device__ __forceinline__ CTab fft_tab(const RunDev &R, int off)
+{
+    return CTab{R.dft_tw + 2 * (size_t)off};
+}
+
+// Forward DFT of length N of buf[0 .. N) in place (the caller has written
+// buf and synchronised; tw holds the twiddles of the FFT length P, P = N
+// for a power of two, else Bluestein's: ch the chirps w_n, bh the filter
+// spectrum).  Point n of every pass is handled by thread n mod kBlock.
+template <int LMAX>
+__device__ void dft_lds(double2 *buf, const double2 *tw, int N, int P, CTab ch, CTab bh)
+{
+    const int T = kBlock, lt = threadIdx.x;
+    if (N == P) {
+        fft_lds<LMAX>(buf, tw, N);
+        return;
+    }
+    constexpr int kE = FftDims<LMAX>::kE;
+    double2 w[kE], h[kE];
+#pragma unroll
+    for (int u = 0; u < kE; u++) {
+        const int n = lt + u * T;
+        w[u] = ch[min(n, N - 1)];
+        h[u] = bh[min(n, P - 1)];
+    }
+#pragma unroll
+    for (int u = 0; u < kE; u++) {
+        const int n = lt + u * T;
+        if (n < P) buf[n] = n < N ? cmul(buf[n], w[u]) : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    fft_lds<LMAX>(buf, tw, P);
+#pragma unroll
+    for (int u = 0; u < kE; u++) {
+        const int k = lt + u * T;
+        if (k < P) {
+            const double2 z = cmul(buf[k], h[u]);
+            buf[k] = make_double2(z.x, -z.y);
+        }
+    }
+    __syncthreads();
+    fft_lds<LMAX>(buf, tw, P);
+#pragma unroll
+    for (int u = 0; u < kE; u++) {
+        const int n = lt + u * T;
+        if (n < N) {
+            const double2 z = buf[n];
+            buf[n] = cmul(make_double2(z.x, -z.y), w[u]);
+        }
+    }
+    __syncthreads();
+}
+
+// The twiddles of FFT length P into LDS, loaded by the whole workgroup
+// (kT per thread) before any LDS store of the caller.
+template <int LMAX>
+struct TwLoad {
+    static constexpr int kT = LMAX / kBlock;
+    double2 t[kT];
+    __device__ __forceinline__ void load(CTab twg, int P)
+    {
+#pragma unroll
+        for (int u = 0; u < kT; u++) t[u] = twg[min((int)threadIdx.x + u * kBlock, P - 1)];
+    }
+    __device__ __forceinline__ void store(double2 *tw, int P) const
+    {
+#pragma unroll
+        for (int u = 0; u < kT; u++)
+            if ((int)threadIdx.x + u * kBlock < P) tw[threadIdx.x + u * kBlock] = t[u];
+    }
+};
+
+// The global loads of a kernel are all issued before the first LDS store
+// (kE points per thread in registers), so that they cost one round trip.
+
+// step 0: row pair -> the two rows' real spectra, column-major in dft_z
 template <int LMAX>
 __device__ void fft_rows_fwd(const RunDev &R, int g, double2 *buf, double2 *tw)
 {
-    constexpr int kE = LMAX / kBlock;
+    constexpr int kE = FftDims<LMAX>::kE;
     const int N = R.ncx, ncy = R.ncy, m = ncy - 1;
+    const FftTabs T = fft_tabs(N, ncy);
+    const int P = T.lx, lt = threadIdx.x;
     const int r1 = 2 * g, r2 = r1 + 1;
     const CUPIC_GLOBAL double *b = gp(R.b64);
     const CUPIC_GLOBAL double *phi = gp(R.phi64);
-    const CUPIC_GLOBAL double *twg = gp(R.dft_tw);
     // rows r1 and r2 (r2 clamped: row m does not exist and reads as 0) and
     // the Dirichlet rows, loaded without conditions so that no load waits
     // for another; the Dirichlet values are added to rows 0 and m - 1
     const int ra = r1 + 1, rb = min(r2, m - 1) + 1;
-    double2 t[kE], v[kE], pd[kE];
+    TwLoad<LMAX> tl;
+    tl.load(fft_tab(R, T.tw_x), P);
+    double2 v[kE], pd[kE];
 #pragma unroll
     for (int u = 0; u < kE; u++) {
-        const int i = min((int)threadIdx.x + u * kBlock, N - 1);
-        t[u] = make_double2(twg[2 * i], twg[2 * i + 1]);
+        const int i = min(lt + u * kBlock, N - 1);
         v[u] = make_double2(b[(size_t)ra * N + i], b[(size_t)rb * N + i]);
         pd[u] = make_double2(phi[i], phi[(size_t)ncy * N + i]);
     }
@@ -404,20 +521,18 @@ __device__ void fft_rows_fwd(const RunDev &R, int g, double2 *buf, double2 *tw)
         if (r2 == m - 1) y += pd[u].y;
         v[u] = make_double2(x, y);
     }
+    tl.store(tw, P);
 #pragma unroll
     for (int u = 0; u < kE; u++) {
-        const int i = threadIdx.x + u * kBlock;
-        if (i < N) {
-            tw[i] = t[u];
-            buf[i] = v[u];
-        }
+        const int i = lt + u * kBlock;
+        if (i < N) buf[i] = v[u];
     }
     __syncthreads();
-    fft_lds<LMAX>(buf, tw, N);
+    dft_lds<LMAX>(buf, tw, N, P, fft_tab(R, T.ch_x), fft_tab(R, T.bh_x));
     const int kc = dft_cos_cols(N), ks = N - kc;
     CUPIC_GLOBAL double *Z = gp(R.dft_z);
-    for (int k = threadIdx.x; k <= N / 2; k += kBlock) {
-        const double2 a = buf[k], c = buf[(N - k) & (N - 1)];
+    for (int k = lt; k <= N / 2; k += kBlock) {
+        const double2 a = buf[k], c = buf[k == 0 ? 0 : N - k];
         const double x1r = 0.5 * (a.x + c.x), x1i = 0.5 * (a.y - c.y);
         const double x2r = 0.5 * (a.y + c.y), x2i = -0.5 * (a.x - c.x);
         Z[(size_t)k * m + r1] = x1r;
@@ -429,92 +544,92 @@ __device__ void fft_rows_fwd(const RunDev &R, int g, double2 *buf, double2 *tw)
     }
 }
 
-// step 1: column pair g: Y = S R^ / mu, then X^ = S Y, column-major in dft_y
+// step 1: column pair: Y = S R^ / mu, then X^ = S Y, column-major in dft_y
 template <int LMAX>
 __device__ void fft_cols(const RunDev &R, int g, double2 *buf, double2 *tw)
 {
-    constexpr int kE = LMAX / kBlock;
-    constexpr int kU = (LMAX / 2 + kBlock - 1) / kBlock;  // m < LMAX / 2
+    constexpr int kE = FftDims<LMAX>::kE, kU = FftDims<LMAX>::kU;
     const int N = R.ncx, M = R.ncy, m = M - 1, L = 2 * M;
+    const FftTabs T = fft_tabs(N, M);
+    const int P = T.ly, lt = threadIdx.x;
     const int c1 = 2 * g, c2 = c1 + 1;
-    const CUPIC_GLOBAL double *twg = gp(R.dft_tw) + 2 * N;
+    const bool two = c2 < N;        // an odd ncx leaves the last column alone
+    const int c2r = two ? c2 : c1;  // its partner reads column c1 and is not stored
     const CUPIC_GLOBAL double *Z = gp(R.dft_z);
     const CUPIC_GLOBAL double *D = gp(R.dft_d);
     const double sc = sqrt(2.0 / (double)M);
     // the odd extension (0, v_0 .. v_{m-1}, 0, -v_{m-1} .. -v_0) of the
     // column values v_j = x_j + i y_j: element n < M is v_{n-1}, element
     // n > M is -v_{L-n-1}
-    double2 t[kE], v[kE];
+    TwLoad<LMAX> tl;
+    tl.load(fft_tab(R, T.tw_y), P);
+    double2 v[kE];
     double dl[kU];
-    const double d1 = D[c1], d2 = D[c2];
+    const double d1 = D[c1], d2 = D[c2r];
 #pragma unroll
     for (int u = 0; u < kE; u++) {
-        const int n = min((int)threadIdx.x + u * kBlock, L - 1);
+        const int n = min(lt + u * kBlock, L - 1);
         const int j = min(max(n < M ? n - 1 : L - n - 1, 0), m - 1);  // clamped for n = 0, M
-        t[u] = make_double2(twg[2 * n], twg[2 * n + 1]);
-        v[u] = make_double2(Z[(size_t)c1 * m + j], Z[(size_t)c2 * m + j]);
+        v[u] = make_double2(Z[(size_t)c1 * m + j], Z[(size_t)c2r * m + j]);
     }
 #pragma unroll
+    for (int u = 0; u < kU; u++) dl[u] = D[N + min(lt + u * kBlock, m - 1)];
+#pragma unroll
     for (int u = 0; u < kE; u++) {
-        const int n = threadIdx.x + u * kBlock;
+        const int n = lt + u * kBlock;
         v[u] = (n == 0 || n == M) ? make_double2(0.0, 0.0)
                                   : (n < M ? v[u] : make_double2(-v[u].x, -v[u].y));
     }
-#pragma unroll
-    for (int u = 0; u < kU; u++) {
-        const int l = threadIdx.x + u * kBlock;
-        if (l < m) dl[u] = D[N + l];
-    }
+    tl.store(tw, P);
 #pragma unroll
     for (int u = 0; u < kE; u++) {
-        const int n = threadIdx.x + u * kBlock;
-        if (n < L) {
-            tw[n] = t[u];
-            buf[n] = v[u];
-        }
+        const int n = lt + u * kBlock;
+        if (n < L) buf[n] = v[u];
     }
     __syncthreads();
-    fft_lds<LMAX>(buf, tw, L);
+    const CTab ch = fft_tab(R, T.ch_y), bh = fft_tab(R, T.bh_y);
+    dft_lds<LMAX>(buf, tw, L, P, ch, bh);
     // coefficient l + 1 is -2i X_l + 2 Y_l: X_l = -Im / 2, Y_l = Re / 2
     double2 s[kU];
 #pragma unroll
     for (int u = 0; u < kU; u++) {
-        const int l = threadIdx.x + u * kBlock;
+        const int l = lt + u * kBlock;
         if (l < m) {
             const double2 V = buf[l + 1];
             s[u] = make_double2((sc * (-0.5 * V.y)) / (d1 + dl[u]), (sc * (0.5 * V.x)) / (d2 + dl[u]));
         }
     }
     __syncthreads();
-    if (threadIdx.x == 0) buf[0] = buf[M] = make_double2(0.0, 0.0);
+    if (lt == 0) buf[0] = buf[M] = make_double2(0.0, 0.0);
 #pragma unroll
     for (int u = 0; u < kU; u++) {
-        const int l = threadIdx.x + u * kBlock;
+        const int l = lt + u * kBlock;
         if (l < m) {
             buf[l + 1] = s[u];
             buf[L - l - 1] = make_double2(-s[u].x, -s[u].y);
         }
     }
     __syncthreads();
-    fft_lds<LMAX>(buf, tw, L);
+    dft_lds<LMAX>(buf, tw, L, P, ch, bh);
     CUPIC_GLOBAL double *Y = gp(R.dft_y);
-    for (int j = threadIdx.x; j < m; j += kBlock) {
+    for (int j = lt; j < m; j += kBlock) {
         const double2 V = buf[j + 1];
         Y[(size_t)c1 * m + j] = sc * (-0.5 * V.y);
-        Y[(size_t)c2 * m + j] = sc * (0.5 * V.x);
+        if (two) Y[(size_t)c2 * m + j] = sc * (0.5 * V.x);
     }
 }
 
-// step 2: row pair g of X^ -> rows r1 + 1, r2 + 1 of the FP64 potential and
+// step 2: row pair of X^ -> rows r1 + 1, r2 + 1 of the FP64 potential and
 // its FP32 copy
 template <int LMAX>
 __device__ void fft_rows_inv(const RunDev &R, const RunState *st, int g, double2 *buf,
                              double2 *tw)
 {
-    constexpr int kE = LMAX / kBlock;
+    constexpr int kE = FftDims<LMAX>::kE;
     const int N = R.ncx, m = R.ncy - 1;
+    const FftTabs T = fft_tabs(N, R.ncy);
+    const int P = T.lx, lt = threadIdx.x;
     const int r1 = 2 * g, r2 = r1 + 1;
-    const CUPIC_GLOBAL double *twg = gp(R.dft_tw);
     const CUPIC_GLOBAL double *X = gp(R.dft_y);
     const int kc = dft_cos_cols(N), ks = N - kc;
     // coefficient k of row r as a complex number (Hermitian beyond N / 2)
@@ -522,20 +637,21 @@ __device__ void fft_rows_inv(const RunDev &R, const RunState *st, int g, double2
     // coefficient kk and the sin coefficient kk (index clamped, zeroed
     // outside 1 .. ks), loaded without conditions
     const int rb = min(r2, m - 1);
-    double2 t[kE], v[kE];
+    TwLoad<LMAX> tl;
+    tl.load(fft_tab(R, T.tw_x), P);
+    double2 v[kE];
     double4 q[kE];
 #pragma unroll
     for (int u = 0; u < kE; u++) {
-        const int k = min((int)threadIdx.x + u * kBlock, N - 1);
+        const int k = min(lt + u * kBlock, N - 1);
         const int kk = k <= N / 2 ? k : N - k;
         const int cs = kc + min(max(kk, 1), max(ks, 1)) - 1;
-        t[u] = make_double2(twg[2 * k], twg[2 * k + 1]);
         q[u] = make_double4(X[(size_t)kk * m + r1], X[(size_t)cs * m + r1], X[(size_t)kk * m + rb],
                             X[(size_t)cs * m + rb]);
     }
 #pragma unroll
     for (int u = 0; u < kE; u++) {
-        const int k = min((int)threadIdx.x + u * kBlock, N - 1);
+        const int k = min(lt + u * kBlock, N - 1);
         const int kk = k <= N / 2 ? k : N - k;
         const bool sin_ok = kk >= 1 && kk <= ks;
         const double sg = k <= N / 2 ? 1.0 : -1.0;  // Hermitian: conj beyond N / 2
@@ -544,22 +660,20 @@ __device__ void fft_rows_inv(const RunDev &R, const RunState *st, int g, double2
         if (r2 >= m) b = make_double2(0.0, 0.0);
         v[u] = make_double2(a.x - b.y, -(a.y + b.x));  // conj(a + i b)
     }
+    tl.store(tw, P);
 #pragma unroll
     for (int u = 0; u < kE; u++) {
-        const int k = threadIdx.x + u * kBlock;
-        if (k < N) {
-            tw[k] = t[u];
-            buf[k] = v[u];
-        }
+        const int k = lt + u * kBlock;
+        if (k < N) buf[k] = v[u];
     }
     __syncthreads();
-    fft_lds<LMAX>(buf, tw, N);
-    const double inv = 1.0 / (double)N;  // a power of two: exact
+    dft_lds<LMAX>(buf, tw, N, P, fft_tab(R, T.ch_x), fft_tab(R, T.bh_x));
+    const double dn = (double)N;  // division: exact scaling for a power of two
     CUPIC_GLOBAL double *phi = gp(R.phi64);
     CUPIC_GLOBAL float *phi32 = gp(R.phi32[st->phi_cur]);
-    for (int i = threadIdx.x; i < N; i += kBlock) {
+    for (int i = lt; i < N; i += kBlock) {
         const double2 z = buf[i];
-        const double v1 = z.x * inv, v2 = -z.y * inv;
+        const double v1 = z.x / dn, v2 = -z.y / dn;
         phi[(size_t)(r1 + 1) * N + i] = v1;
         phi32[(size_t)(r1 + 1) * N + i] = (float)v1;
         if (r2 < m) {
@@ -569,8 +683,38 @@ __device__ void fft_rows_inv(const RunDev &R, const RunState *st, int g, double2
     }
 }
 
+// step 3 (set-up, after k_dft_init): block 0 the x filter spectrum, block 1
+// the y one, FFT(b) / P with b_j = conj w_j for j < N, b_(P-j) = conj w_j
+// for 0 < j < N and zero between (P >= 2N - 1: the two ranges are
+// disjoint); nothing for a power-of-two length
 template <int LMAX>
-__global__ __launch_bounds__(kBlock) void k_fft(const RunDev *rds, const int *start, int nruns,
+__device__ void fft_filter(const RunDev &R, int dirn, double2 *buf, double2 *tw)
+{
+    const FftTabs T = fft_tabs(R.ncx, R.ncy);
+    const int N = dirn == 0 ? R.ncx : 2 * R.ncy;
+    const int P = dirn == 0 ? T.lx : T.ly;
+    if (N == P) return;
+    const CTab twg = fft_tab(R, dirn == 0 ? T.tw_x : T.tw_y);
+    const CTab ch = fft_tab(R, dirn == 0 ? T.ch_x : T.ch_y);
+    for (int j = threadIdx.x; j < P; j += kBlock) {
+        tw[j] = twg[j];
+        const int n = j < N ? j : (j > P - N ? P - j : -1);
+        double2 w = make_double2(0.0, 0.0);
+        if (n >= 0) w = ch[n];
+        buf[j] = make_double2(w.x, -w.y);
+    }
+    __syncthreads();
+    fft_lds<LMAX>(buf, tw, P);
+    double2 *bh = reinterpret_cast<double2 *>(R.dft_tw) + (dirn == 0 ? T.bh_x : T.bh_y);
+    const double dp = (double)P;  // a power of two: exact
+    for (int k = threadIdx.x; k < P; k += kBlock) {
+        const double2 z = buf[k];
+        bh[k] = make_double2(z.x / dp, z.y / dp);
+    }
+}
+
+template <int LMAX>
+__global__ __launch_bounds__(kBlock, LMAX <= 1024 ? 4 : (LMAX <= 2048 ? 2 : 1)) void k_fft(const RunDev *rds, const int *start, int nruns,
                                                 int step)
 {
     RUN_PROLOGUE();
@@ -580,8 +724,10 @@ __global__ __launch_bounds__(kBlock) void k_fft(const RunDev *rds, const int *st
         fft_rows_fwd<LMAX>(R, local, buf, tw);
     else if (step == 1)
         fft_cols<LMAX>(R, local, buf, tw);
-    else
+    else if (step == 2)
         fft_rows_inv<LMAX>(R, st, local, buf, tw);
+    else
+        fft_filter<LMAX>(R, local, buf, tw);
 }
 
 #define LAUNCH_K(kern, L, ...)                                                         \
@@ -594,12 +740,14 @@ __global__ __launch_bounds__(kBlock) void k_fft(const RunDev *rds, const int *st
 void launch_dft_init(const Launch &L) { LAUNCH_K(k_dft_init, L); }
 void launch_dft_gemm(const Launch &L, int step) { LAUNCH_K(k_dft_gemm, L, step); }
 void launch_dft_reduce(const Launch &L, int step) { LAUNCH_K(k_dft_reduce, L, step); }
-void launch_fft(const Launch &L, int step, bool large)
+void launch_fft(const Launch &L, int step, int cls)
 {
-    if (large)
-        LAUNCH_K(k_fft<kFftL>, L, step);
-    else
+    if (cls == 1)
         LAUNCH_K(k_fft<kFftS>, L, step);
+    else if (cls == 2)
+        LAUNCH_K(k_fft<kFftM>, L, step);
+    else
+        LAUNCH_K(k_fft<kFftL>, L, step);
 }
 
 }  // namespace cupic

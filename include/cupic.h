@@ -45,9 +45,17 @@ extern "C" {
 #define CUPIC_JACOBI_RESIDUAL 1   /* FP32 sweeps, FP64 residual checked every check_every     */
 #define CUPIC_JACOBI_REFINE 2     /* FP64 residual + FP32 correction solve (refinement)        */
 #define CUPIC_POISSON_DIRECT 3    /* the system Jacobi converges to, solved directly in FP64:
-                                     DFT along periodic x, sine transform along y (FFT when
-                                     ncx and 2 ncy are powers of two up to 4096, else dense
-                                     FP64 products)                                          */
+                                     DFT along periodic x, sine transform along y, by the
+                                     method direct_method selects                           */
+
+/* direct_method: how CUPIC_POISSON_DIRECT transforms */
+#define CUPIC_DIRECT_AUTO 0   /* FFT kernels when ncx and 2 ncy are powers of two up to 4096,
+                                 else the dense FP64 matrix products                    */
+#define CUPIC_DIRECT_DENSE 1  /* always the dense FP64 matrix products                  */
+#define CUPIC_DIRECT_FFT 2    /* FFT kernels for any ncx and ncy whose FFT lengths are at
+                                 most 4096: a power-of-two length n as one FFT, another
+                                 length by Bluestein's chirp z-transform through two FFTs
+                                 of the power of two P >= 2n - 1; else the dense products */
 
 /* quirk flags */
 #define CUPIC_QUIRK_Q7_REFERENCE_KICK 1u /* injection half-kick sign as committed (src/cc.cu:243-244) */
@@ -71,6 +79,7 @@ typedef struct cupic_params {
     int32_t tile_cells;      /* cells per wave strip in the push passes: 0 picks
                                 round(2048 / ppc), 1..16 fixes it (results do not
                                 depend on it; see cupic_autotune_tile)           */
+    int32_t direct_method;   /* CUPIC_DIRECT_*: the transforms of CUPIC_POISSON_DIRECT */
     double jacobi_tol;       /* per-step tolerance (reference 1e-4, src/main.cu:64) */
     double jacobi_tol_init;  /* init tolerance (reference 1e-3, src/init.cu:254)     */
     /* derived by cupic_params_derive */

@@ -81,6 +81,24 @@ def test_invalid_params_rejected():
         cp.params(dt=-1.0)
     with pytest.raises(cp.CupicError):
         cp.params(jacobi_mode=7)
+    with pytest.raises(cp.CupicError):
+        cp.params(direct_method=3)
+    for m in (cp.DIRECT_AUTO, cp.DIRECT_DENSE, cp.DIRECT_FFT):
+        assert cp.params(direct_method=m).direct_method == m
+
+
+def test_direct_solve_footprint_by_method():
+    """The direct solve's tables: the dense products hold two ncx x ncx and
+    one m x m FP64 bases, the FFT path a few rows of twiddles, chirps and
+    filter spectra, so on a large grid the dense footprint is far above the
+    FFT one, whatever the lengths."""
+    import cupic_amd as cp
+    kw = dict(ncx=1000, ncy=500, ppc=1, jacobi_mode=cp.POISSON_DIRECT)
+    dense = cp.footprint(cp.params(**kw, direct_method=cp.DIRECT_DENSE))
+    auto = cp.footprint(cp.params(**kw))            # not powers of two: dense
+    fft = cp.footprint(cp.params(**kw, direct_method=cp.DIRECT_FFT))
+    assert auto == dense
+    assert dense - fft > 8 * (2 * 1000 * 1000 + 499 * 499) * 0.9
 
 
 def test_footprint_scales_with_particles():
