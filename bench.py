@@ -352,7 +352,7 @@ def converged_rule(prev_net, ie, ii, rel):
     return gross > 0.0 and abs((ii - ie) - prev_net) / gross < rel
 
 
-def serving_leg(cp, n_req, rate, max_steps, seed, plain="full"):
+def serving_leg(cp, n_req, rate, max_steps, seed, plain="full", max_particles=0, policy=0):
     """Continuous batching (cupic_server) against the same points served one
     at a time through the plain path with the same convergence rule.  plain
     "full" steps every point through the plain path and checks that the
@@ -363,7 +363,7 @@ def serving_leg(cp, n_req, rate, max_steps, seed, plain="full"):
     reqs = draw_requests(n_req, rate, seed)
     window, rnd, rel = 500, 50, 0.005
     srv_cfg = cp.server_config(max_steps=max_steps, window_steps=window, round_steps=rnd,
-                               rel_change=rel)
+                               rel_change=rel, max_particles=max_particles, policy=policy)
     srv = cp.Server(srv_cfg)
     start = srv.stats()["clock"]
     pending = list(range(n_req))
@@ -474,6 +474,8 @@ def serving_leg(cp, n_req, rate, max_steps, seed, plain="full"):
                        f"the gross current between {window}-step windows (cap {max_steps} "
                        f"steps); direct FP64 Poisson solve",
            "direct_method": ("auto", "dense", "fft")[DIRECT_METHOD],
+           "admission": {"policy": ("fcfs", "smallest_first")[policy],
+                         "max_resident_particles": max_particles or None},
            "served": served, "plain_one_at_a_time": plain_res,
            "points_per_s_vs_plain": served["points_per_s"] / plain_res["points_per_s"]}
     if plain == "full":
@@ -513,6 +515,9 @@ def main():
                          "power-of-two lengths else dense products (auto), dense products "
                          "always, or FFT for any length (Bluestein)")
     ap.add_argument("--serve-requests", type=int, default=50)
+    ap.add_argument("--serve-max-particles", type=float, default=3e8,
+                    help="server's resident-particle budget (0: none)")
+    ap.add_argument("--serve-policy", default="fcfs", choices=["fcfs", "smallest"])
     ap.add_argument("--serve-rate", type=float, default=50.0)
     ap.add_argument("--serve-max-steps", type=int, default=10000)
     ap.add_argument("--serve-plain", default="estimate", choices=["full", "estimate"],
@@ -545,7 +550,8 @@ def main():
               file=sys.stderr, flush=True)
     if a.serving_only:
         res = serving_leg(cp, a.serve_requests, a.serve_rate, a.serve_max_steps, 20261203,
-                          a.serve_plain)
+                          a.serve_plain, int(a.serve_max_particles),
+                          1 if a.serve_policy == "smallest" else 0)
         print(json.dumps({"serving": res}), flush=True)
         return
     if a.batched_only:
@@ -719,7 +725,8 @@ def main():
         print(f"# batched: {out['batched']}", file=sys.stderr, flush=True)
     if a.serve_requests > 0:
         out["serving"] = serving_leg(cp, a.serve_requests, a.serve_rate, a.serve_max_steps,
-                                     20261203, a.serve_plain)
+                                     20261203, a.serve_plain, int(a.serve_max_particles),
+                                     1 if a.serve_policy == "smallest" else 0)
     if not a.no_cpu and a.cpu_max_steps > 0:
         print("# cpu baseline ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(cfg, a.cpu_seconds, a.cpu_max_steps, solver)

@@ -68,7 +68,7 @@ class ServerConfig(C.Structure):
         ("window_steps", C.c_int32), ("max_steps", C.c_int32), ("rel_change", C.c_double),
         ("use_graph", C.c_int32), ("jacobi_mode", C.c_int32),
         ("jacobi_max_sweeps", C.c_int32), ("jacobi_check_every", C.c_int32),
-        ("jacobi_tol", C.c_double),
+        ("jacobi_tol", C.c_double), ("max_particles", C.c_int64), ("policy", C.c_int32),
     ]
 
 
@@ -89,6 +89,7 @@ class Point(C.Structure):
 
 
 POINT_QUEUED, POINT_RUNNING, POINT_CONVERGED, POINT_CAPPED, POINT_FAILED = 0, 1, 2, 3, 4
+SERVE_FCFS, SERVE_SMALLEST_FIRST = 0, 1
 
 
 def _is_product_library(path):
@@ -622,8 +623,8 @@ def server_config(**kw):
 class Server:
     """Continuous batching of I-V sweep points (one request = one bias point):
     resident runs advance together in batched rounds until their probe current
-    is steady; requests are admitted first come first served by slots and
-    device-memory budget."""
+    is steady; requests are admitted in arrival order (or fewest particles
+    first) by slots, device-memory budget and resident-particle budget."""
 
     def __init__(self, cfg=None, **kw):
         self.cfg = cfg if cfg is not None else server_config(**kw)

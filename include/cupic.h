@@ -306,7 +306,17 @@ typedef struct cupic_server_config {
     int32_t jacobi_max_sweeps;
     int32_t jacobi_check_every;
     double jacobi_tol;
+    int64_t max_particles;     /* resident particles (initial load, both
+                                  species) at most, 0: no limit; one run is
+                                  always admitted into an empty server     */
+    int32_t policy;            /* CUPIC_SERVE_*: which queued point next      */
 } cupic_server_config;
+
+/* admission order of queued points */
+#define CUPIC_SERVE_FCFS 0            /* arrival order                          */
+#define CUPIC_SERVE_SMALLEST_FIRST 1  /* fewest particles first (arrival order
+                                         among equals); a large point waits
+                                         while smaller ones keep arriving     */
 
 #define CUPIC_POINT_QUEUED 0
 #define CUPIC_POINT_RUNNING 1
@@ -331,13 +341,15 @@ typedef struct cupic_point {
 
 typedef struct cupic_server cupic_server;
 
-/* defaults: 200 GiB, 256 runs, rounds of 50 steps, 500-step windows, cap
- * 20000 steps, rel_change 0.005, graphs on, CUPIC_POISSON_DIRECT */
+/* defaults: 200 GiB, 256 runs, 300 M resident particles, first come first
+ * served, rounds of 50 steps, 500-step windows, cap 20000 steps, rel_change
+ * 0.005, graphs on, CUPIC_POISSON_DIRECT */
 int cupic_server_config_default(cupic_server_config *cfg);
 int cupic_server_create(const cupic_server_config *cfg, cupic_server **out);
 /* queues a point; the Jacobi settings of the config replace those of p */
 int cupic_server_submit(cupic_server *s, const cupic_params *p, int64_t *id);
-/* one round: admit, advance every resident run by round_steps, retire */
+/* one round: admit (slots, HBM budget, particle budget, in the policy's
+ * order), advance every resident run by round_steps, retire */
 int cupic_server_poll(cupic_server *s, int32_t *n_active, int32_t *n_queued);
 int cupic_server_result(cupic_server *s, int64_t id, cupic_point *out);
 int cupic_server_stats(cupic_server *s, int64_t *rounds, int64_t *rebuilds, int64_t *hbm_used,

@@ -1,7 +1,8 @@
 """Serving (continuous batching of I-V points) on the GPU, through the C-ABI:
 a point's result is the same alone and in a mix that it joins mid-way (bit
 for bit: probe counts, particle counts, window currents, steps); admission
-respects the device-memory budget; statuses and step accounting."""
+respects the device-memory and resident-particle budgets and the admission
+order; statuses and step accounting."""
 import sys
 
 import pytest
@@ -85,6 +86,29 @@ def test_admission_respects_budget(dev):
     for r in res:
         assert r["steps"] == 200 or r["status"] == cp.POINT_CONVERGED
     srv.close()
+
+
+def test_particle_budget_and_admission_order(dev):
+    """With a resident-particle budget that holds one point at a time,
+    points run in turn: in arrival order (FCFS), or fewest particles first;
+    a point larger than the budget is still admitted into an empty server."""
+    sizes = [(48, 24, 10), (32, 16, 4), (40, 20, 8)]
+    ps = [point(nx, ny, ppc, -3.0, 20 + k) for k, (nx, ny, ppc) in enumerate(sizes)]
+    npart = [2 * p.per_row * p.ncy for p in ps]
+    for policy, order in ((cp.SERVE_FCFS, [0, 1, 2]),
+                          (cp.SERVE_SMALLEST_FIRST, sorted(range(3), key=lambda k: npart[k]))):
+        budget = min(npart[i] + npart[j] for i in range(3) for j in range(3) if i != j) - 1
+        srv = cp.Server(cfg(max_particles=budget, max_steps=100, policy=policy))
+        ids = [srv.submit(p) for p in ps]
+        active, queued = srv.poll()
+        assert (active, queued) == (1, 2)
+        res = drain(srv, ids)
+        srv.close()
+        admitted = sorted(range(3), key=lambda k: res[k]["t_admit"])
+        say("policy", policy, "admission order", admitted, "particles", npart)
+        assert admitted == order
+        for a, b in zip(order, order[1:]):
+            assert res[b]["t_admit"] >= res[a]["t_done"]
 
 
 def test_request_over_budget_is_rejected(dev):
