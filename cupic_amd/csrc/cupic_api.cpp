@@ -82,7 +82,8 @@ struct cupic_run {
 struct cupic_batch {
     std::vector<cupic_run *> runs;
     RunDev *d_rds = nullptr;
-    int *d_plan = nullptr;   // kShapeCount * (nruns + 1)
+    int *d_plan = nullptr;   // kShapeCount rows: block prefixes, then the block map
+    int64_t plan_off[kShapeCount] = {};   // row s of d_plan
     int nblk[kShapeCount] = {};
     hipStream_t stream = nullptr;
     hipStream_t side = nullptr;
@@ -96,9 +97,6 @@ struct cupic_batch {
 
 namespace cupic {
 
-// Direct solve: output tiles of one product over the m = ncy - 1 interior
-// rows and ncx columns, and the k slices that bring a product near
-// kGSlices workgroups (never more slices than k tiles).
 // The direct solve's method for a run: its FFT instance (fft_class), or 0
 // for the dense products.
 int direct_class(const cupic_params &p)
@@ -108,6 +106,9 @@ int direct_class(const cupic_params &p)
                : fft_class(p.ncx, p.ncy, p.direct_method == CUPIC_DIRECT_FFT);
 }
 
+// Direct solve: output tiles of one product over the m = ncy - 1 interior
+// rows and ncx columns, and the k slices that bring a product near
+// kGSlices workgroups (never more slices than k tiles).
 int dft_tiles(int ncx, int m) { return ((m + kGM - 1) / kGM) * ((ncx + kGN - 1) / kGN); }
 int dft_slices(int ncx, int m, int Kd)
 {
@@ -177,7 +178,8 @@ namespace {
 // A sequence target: a single run or a batch.
 struct Target {
     const RunDev *rds;
-    const int *plan;      // kShapeCount rows of (nruns + 1)
+    const int *plan;      // kShapeCount rows: row s at off[s], or at 2 s for a single run
+                          // (block prefixes 0, nblk[s])
     const int *nblk;
     int nruns;
     hipStream_t stream;
@@ -185,13 +187,14 @@ struct Target {
     hipStream_t side = nullptr;   // injection clock runs here when set
     hipEvent_t fork = nullptr, join = nullptr, field = nullptr;
     bool seg_stayers = false;     // scatter instance for low particles per cell
+    const int64_t *off = nullptr;  // batch plan rows (null: a single run's)
     Launch L(Shape s) const
     {
-        return Launch{rds, plan + (size_t)s * (nruns + 1), nruns, nblk[s], stream};
+        return Launch{rds, plan + (off ? off[s] : 2 * (int64_t)s), nruns, nblk[s], stream};
     }
     Launch Lside(Shape s) const
     {
-        return Launch{rds, plan + (size_t)s * (nruns + 1), nruns, nblk[s], side};
+        return Launch{rds, plan + (off ? off[s] : 2 * (int64_t)s), nruns, nblk[s], side};
     }
 };
 
@@ -1567,6 +1570,7 @@ namespace {
 Target batch_target(cupic_batch *b)
 {
     Target T{b->d_rds, b->d_plan, b->nblk, (int)b->runs.size(), b->stream, nullptr};
+    T.off = b->plan_off;
     T.side = b->side;
     T.fork = b->ev_fork;
     T.join = b->ev_join;
@@ -1650,15 +1654,32 @@ int cupic_batch_create(cupic_run *const *runs, int32_t nruns, int32_t use_graph,
         st = write_state(r, s);
         if (st) { cupic_batch_destroy(b); return st; }
     }
-    std::vector<int> plan((size_t)kShapeCount * (nruns + 1));
+    // row s: nruns + 1 block prefixes, then from plan_map_at(nruns) the
+    // (run, local block) pair of each of the row's blocks
+    std::vector<int> nbs((size_t)kShapeCount * nruns);
+    size_t total = 0;
     for (int s = 0; s < kShapeCount; s++) {
+        b->plan_off[s] = (int64_t)total;
+        int acc = 0;
+        for (int k = 0; k < nruns; k++) acc += (nbs[(size_t)s * nruns + k] = shape_blocks(hs[k], (Shape)s));
+        b->nblk[s] = acc;
+        total += plan_map_at(nruns) + 2 * (size_t)acc;
+    }
+    std::vector<int> plan(total, 0);
+    for (int s = 0; s < kShapeCount; s++) {
+        int *row = plan.data() + b->plan_off[s];
+        int *map = row + plan_map_at(nruns);
         int acc = 0;
         for (int k = 0; k < nruns; k++) {
-            plan[(size_t)s * (nruns + 1) + k] = acc;
-            acc += shape_blocks(hs[k], (Shape)s);
+            row[k] = acc;
+            const int nb = nbs[(size_t)s * nruns + k];
+            for (int q = 0; q < nb; q++) {
+                map[2 * (acc + q)] = k;
+                map[2 * (acc + q) + 1] = q;
+            }
+            acc += nb;
         }
-        plan[(size_t)s * (nruns + 1) + nruns] = acc;
-        b->nblk[s] = acc;
+        row[nruns] = acc;
     }
     if (hipMalloc(&b->d_rds, hs.size() * sizeof(RunDev)) != hipSuccess ||
         hipMalloc(&b->d_plan, plan.size() * sizeof(int)) != hipSuccess) {

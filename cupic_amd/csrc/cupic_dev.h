@@ -308,15 +308,10 @@ __device__ __forceinline__ float2 ld_f2(const float2 *p, int k)
     return make_float2(v.x, v.y);
 }
 
-__device__ __forceinline__ int find_run(const int *__restrict__ start, int nruns, int b)
-{
-    int lo = 0, hi = nruns - 1;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (start[mid] <= b) lo = mid; else hi = mid - 1;
-    }
-    return lo;
-}
+// A batch launch's plan row: nruns + 1 block prefixes, then, from this
+// (even) offset, the (run, local block) pair of every block, so that a block
+// finds its run with one load instead of a search over the prefixes.
+__host__ __device__ constexpr int plan_map_at(int nruns) { return (nruns + 2) & ~1; }
 
 // FP32 potential buffers of modes 0 and 1.  A solve starts from buffer
 // a = phi_cur and alternates between a and phi_alt(a): after `par` sweeps
@@ -385,10 +380,12 @@ __device__ __forceinline__ void charge_to_rhs(const RunDev &R, long long a, int 
 // start[0] is 0, so a launch for one run needs no load from start.
 #define RUN_PROLOGUE()                                                  \
     const int b_ = blockIdx.x;                                          \
-    const int r_ = __builtin_amdgcn_readfirstlane(                      \
-        nruns == 1 ? 0 : find_run(start, nruns, b_));  /* uniform */    \
+    const int2 rl_ = nruns == 1 ? make_int2(0, b_)                      \
+                                : reinterpret_cast<const int2 *>(       \
+                                      start + plan_map_at(nruns))[b_];  \
+    const int r_ = __builtin_amdgcn_readfirstlane(rl_.x); /* uniform */ \
     const RunDev &R = rds[r_];                                          \
-    const int local = nruns == 1 ? b_ : b_ - start[r_];                 \
+    const int local = __builtin_amdgcn_readfirstlane(rl_.y);            \
     RunState *st = R.st;                                                \
     (void)local; (void)st;
 

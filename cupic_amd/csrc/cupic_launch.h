@@ -13,7 +13,9 @@ namespace cupic {
 
 struct Launch {
     const RunDev *rds;     // device array, nruns entries
-    const int *start;      // device array, nruns + 1 block prefixes
+    const int *start;      // device array, nruns + 1 block prefixes; with
+                           // nruns > 1 followed, from plan_map_at(nruns), by
+                           // the (run, local block) pair of every block
     int nruns;
     int nblocks;
     hipStream_t stream;
