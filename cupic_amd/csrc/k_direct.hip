@@ -431,37 +431,40 @@ __device__ void dft_lds(double2 *buf, const double2 *tw, int N, int P, CTab ch, 
         fft_lds<LMAX>(buf, tw, N);
         return;
     }
+    // the chirps and the filter spectrum are read where they are used (L2
+    // hits), so that nothing stays live across the FFTs: registers are the
+    // kernel's, and power-of-two runs take the same kernel
     constexpr int kE = FftDims<LMAX>::kE;
-    double2 w[kE], h[kE];
+    double2 t[kE];
+#pragma unroll
+    for (int u = 0; u < kE; u++) t[u] = ch[min(lt + u * T, N - 1)];
 #pragma unroll
     for (int u = 0; u < kE; u++) {
         const int n = lt + u * T;
-        w[u] = ch[min(n, N - 1)];
-        h[u] = bh[min(n, P - 1)];
-    }
-#pragma unroll
-    for (int u = 0; u < kE; u++) {
-        const int n = lt + u * T;
-        if (n < P) buf[n] = n < N ? cmul(buf[n], w[u]) : make_double2(0.0, 0.0);
+        if (n < P) buf[n] = n < N ? cmul(buf[n], t[u]) : make_double2(0.0, 0.0);
     }
     __syncthreads();
     fft_lds<LMAX>(buf, tw, P);
 #pragma unroll
+    for (int u = 0; u < kE; u++) t[u] = bh[min(lt + u * T, P - 1)];
+#pragma unroll
     for (int u = 0; u < kE; u++) {
         const int k = lt + u * T;
         if (k < P) {
-            const double2 z = cmul(buf[k], h[u]);
+            const double2 z = cmul(buf[k], t[u]);
             buf[k] = make_double2(z.x, -z.y);
         }
     }
     __syncthreads();
     fft_lds<LMAX>(buf, tw, P);
 #pragma unroll
+    for (int u = 0; u < kE; u++) t[u] = ch[min(lt + u * T, N - 1)];
+#pragma unroll
     for (int u = 0; u < kE; u++) {
         const int n = lt + u * T;
         if (n < N) {
             const double2 z = buf[n];
-            buf[n] = cmul(make_double2(z.x, -z.y), w[u]);
+            buf[n] = cmul(make_double2(z.x, -z.y), t[u]);
         }
     }
     __syncthreads();
@@ -714,7 +717,7 @@ __device__ void fft_filter(const RunDev &R, int dirn, double2 *buf, double2 *tw)
 }
 
 template <int LMAX>
-__global__ __launch_bounds__(kBlock, LMAX <= 1024 ? 4 : (LMAX <= 2048 ? 2 : 1)) void k_fft(const RunDev *rds, const int *start, int nruns,
+__global__ __launch_bounds__(kBlock) void k_fft(const RunDev *rds, const int *start, int nruns,
                                                 int step)
 {
     RUN_PROLOGUE();
