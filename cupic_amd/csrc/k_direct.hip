@@ -671,12 +671,15 @@ __device__ void fft_rows_inv(const RunDev &R, const RunState *st, int g, double2
     }
     __syncthreads();
     dft_lds<LMAX>(buf, tw, N, P, fft_tab(R, T.ch_x), fft_tab(R, T.bh_x));
-    const double dn = (double)N;  // division: exact scaling for a power of two
+    // / N: for a power of two the product with the exact inverse (the same
+    // value, without the division's instruction sequence)
+    const double dn = (double)N, inv = 1.0 / dn;
+    const bool p2 = pow2_len(N);
     CUPIC_GLOBAL double *phi = gp(R.phi64);
     CUPIC_GLOBAL float *phi32 = gp(R.phi32[st->phi_cur]);
     for (int i = lt; i < N; i += kBlock) {
         const double2 z = buf[i];
-        const double v1 = z.x / dn, v2 = -z.y / dn;
+        const double v1 = p2 ? z.x * inv : z.x / dn, v2 = p2 ? -z.y * inv : -z.y / dn;
         phi[(size_t)(r1 + 1) * N + i] = v1;
         phi32[(size_t)(r1 + 1) * N + i] = (float)v1;
         if (r2 < m) {
@@ -709,10 +712,10 @@ __device__ void fft_filter(const RunDev &R, int dirn, double2 *buf, double2 *tw)
     __syncthreads();
     fft_lds<LMAX>(buf, tw, P);
     double2 *bh = reinterpret_cast<double2 *>(R.dft_tw) + (dirn == 0 ? T.bh_x : T.bh_y);
-    const double dp = (double)P;  // a power of two: exact
+    const double ip = 1.0 / (double)P;  // a power of two: exact
     for (int k = threadIdx.x; k < P; k += kBlock) {
         const double2 z = buf[k];
-        bh[k] = make_double2(z.x / dp, z.y / dp);
+        bh[k] = make_double2(z.x * ip, z.y * ip);
     }
 }
 
