@@ -601,15 +601,6 @@ __device__ __forceinline__ void count_task(const RunDev &R, RunState *st, const 
     int qn = 0;
     auto window = [&](int w0, unsigned word) {
         while (cb + 1 < nc && __builtin_amdgcn_readlane(bv, cb + 1) <= w0) cb++;
-        int cl[4];
-#pragma unroll
-        for (int q = 0; q < 4; q++) cl[q] = cb;
-        for (int k = cb + 1; k < nc; k++) {
-            const int bk = __builtin_amdgcn_readlane(bv, k);
-            if (bk >= w0 + 4 * kWave) break;
-#pragma unroll
-            for (int q = 0; q < 4; q++) cl[q] += (w0 + 4 * lane + q >= bk);
-        }
         // The four hint bytes are classified together, one flag per byte
         // (bit 0): valid (slot in the strip), use (code >= c_need, as
         // code + 16 - c_need >= 16), mov (trusted and direction != 4).
@@ -625,10 +616,23 @@ __device__ __forceinline__ void count_task(const RunDev &R, RunState *st, const 
         const unsigned use = (((word >> 4) & 0x0F0F0F0Fu) + code_add) >> 4 & valid;
         const unsigned mov = (((word & 0x0F0F0F0Fu) ^ 0x04040404u) + 0x0F0F0F0Fu) >> 4 & use;
         const unsigned full = valid & ~use;
+        // movers are counted by (strip cell, direction); a window without
+        // any (most of the slow species') skips the cells and the atomics
+        if (__ballot(mov != 0u)) {
+            int cl[4];
 #pragma unroll
-        for (int q = 0; q < 4; q++)
-            if ((mov >> (8 * q)) & 1u)
-                atomicAdd(&cntw[__mul24(cl[q], 9) + (int)((word >> (8 * q)) & 15u)], 1);
+            for (int q = 0; q < 4; q++) cl[q] = cb;
+            for (int k = cb + 1; k < nc; k++) {
+                const int bk = __builtin_amdgcn_readlane(bv, k);
+                if (bk >= w0 + 4 * kWave) break;
+#pragma unroll
+                for (int q = 0; q < 4; q++) cl[q] += (w0 + 4 * lane + q >= bk);
+            }
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+                if ((mov >> (8 * q)) & 1u)
+                    atomicAdd(&cntw[__mul24(cl[q], 9) + (int)((word >> (8 * q)) & 15u)], 1);
+        }
         if (__ballot(full != 0u)) {
 #pragma unroll
             for (int q = 0; q < 4; q++) {
