@@ -131,12 +131,8 @@ int shape_blocks(const RunDev &h, Shape s)
     case kShapeInject: return 2 * ((h.max_inject + kBlock - 1) / kBlock);
     case kShapeRows: return 2 * h.ncy;
     case kShapeInit: return 2 * ((h.n_init + kBlock - 1) / kBlock);
-    case kShapeJacBlk:
-        return ((h.ncx + jac_tx(kJacH) - 1) / jac_tx(kJacH)) *
-               ((h.ncy - 1 + jac_ty(kJacH) - 1) / jac_ty(kJacH));
-    case kShapeJacBlkN:
-        return ((h.ncx + jac_tx(kJacHNarrow) - 1) / jac_tx(kJacHNarrow)) *
-               ((h.ncy - 1 + jac_ty(kJacHNarrow) - 1) / jac_ty(kJacHNarrow));
+    case kShapeJacBlk: return jac_blocks(h.ncx, h.ncy, kJacH);
+    case kShapeJacBlkN: return jac_blocks(h.ncx, h.ncy, kJacHNarrow);
     case kShapeDftInit: {
         if (!h.dft_d) return 0;
         const int64_t m = h.ncy - 1;
@@ -188,6 +184,8 @@ struct Target {
     hipEvent_t fork = nullptr, join = nullptr, field = nullptr;
     bool seg_stayers = false;     // scatter instance for low particles per cell
     const int64_t *off = nullptr;  // batch plan rows (null: a single run's)
+    int jac_h = 0;                // a single run's blocked-Jacobi halo (0: by block count)
+    int jac_nb = 0;               // and its workgroups
     Launch L(Shape s) const
     {
         return Launch{rds, plan + (off ? off[s] : 2 * (int64_t)s), nruns, nblk[s], stream};
@@ -206,6 +204,8 @@ Target target_of(cupic_run *r)
     T.join = r->ev_join;
     T.field = r->ev_field;
     T.seg_stayers = (int64_t)r->p.per_row < (int64_t)kSegStayersPpc * r->p.ncx;
+    T.jac_h = jac_solo_halo(r->h.ncx, r->h.ncy);
+    T.jac_nb = jac_blocks(r->h.ncx, r->h.ncy, T.jac_h);
     return T;
 }
 
@@ -287,9 +287,11 @@ void seq_solve(const Target &T, int use_args, int mode, double tol, int min_swee
             launch_residual(T.L(kShapeResid), 0, 3, 0, 1);  // its last block ends the solve
         return;
     }
-    const bool narrow = T.nblk[kShapeJacBlk] >= kJacNarrowMinBlocks;
-    const int halo = narrow ? kJacHNarrow : kJacH;
+    const bool narrow = T.jac_h > 0 ? T.jac_h == kJacHNarrow
+                                    : T.nblk[kShapeJacBlk] >= kJacNarrowMinBlocks;
+    const int halo = T.jac_h > 0 ? T.jac_h : narrow ? kJacHNarrow : kJacH;
     Launch B = T.L(narrow ? kShapeJacBlkN : kShapeJacBlk);
+    if (T.jac_h > 0) B.nblocks = T.jac_nb;  // a single run: blocks map one to one
     if (check < 1) check = 1;
     const int per = halo - 1;  // sweeps per blocked launch
     int c = 0, s = 0, nlaunch = 0, dpar = 0;

@@ -111,6 +111,24 @@ constexpr int kJacHNarrow = 10;
 constexpr int kJacNarrowMinBlocks = 1024;   // wide-halo workgroups from which to go narrow
 __host__ __device__ constexpr int jac_tx(int h) { return 64 - 2 * h; }
 __host__ __device__ constexpr int jac_ty(int h) { return kJacRows * kJacWaves - 2 * h; }
+// workgroups of a blocked-Jacobi launch with halo h over an ncx x ncy grid
+__host__ __device__ constexpr int jac_blocks(int ncx, int ncy, int h)
+{
+    return ((ncx + jac_tx(h) - 1) / jac_tx(h)) * ((ncy - 1 + jac_ty(h) - 1) / jac_ty(h));
+}
+// A single run whose grid leaves compute units idle takes a wider halo
+// still: the widest of these whose launch has at most one workgroup per
+// compute unit (MI355X: 256), as the redundant work then runs on units that
+// would have none, while each launch saved is about 7 us of fixed cost.
+constexpr int kJacHSolo[3] = {28, 24, 20};
+constexpr int kJacSoloMaxBlocks = 256;
+// The halo of a single run's blocked Jacobi.
+inline int jac_solo_halo(int ncx, int ncy)
+{
+    for (int h : kJacHSolo)
+        if (jac_blocks(ncx, ncy, h) <= kJacSoloMaxBlocks) return h;
+    return jac_blocks(ncx, ncy, kJacH) >= kJacNarrowMinBlocks ? kJacHNarrow : kJacH;
+}
 
 enum Dir : int { kDirFar = 9, kDirProbe = 10, kDirTop = 11, kDirNone = 15 };
 

@@ -547,16 +547,24 @@ void launch_jacobi_block(const Launch &L, int halo, int s0, int nsw, int mode, i
     hipLaunchKernelGGL((k_jacobi_block<HH, RR>), dim3(L.nblocks), dim3(kJacThreads), 0, L.stream, \
                        L.rds, L.start, L.nruns, s0, nsw, mode, first, par, cslot, cat, cpar,    \
                        rslot, rat)
+#define JAC_HALO(HH)                                                                          \
+    do {                                                                                      \
+        if (rk == 2) JAC_LAUNCH(HH, 2);                                                       \
+        else if (rk == 1) JAC_LAUNCH(HH, 1);                                                  \
+        else JAC_LAUNCH(HH, 0);                                                               \
+    } while (0)
+    static_assert(kJacHSolo[0] == 28 && kJacHSolo[1] == 24 && kJacHSolo[2] == 20,
+                  "one instance per halo");
     const int rk = rslot < 0 ? 0 : (rat > 0 && rat < nsw) ? 2 : 1;
-    if (halo == kJacHNarrow) {
-        if (rk == 2) JAC_LAUNCH(kJacHNarrow, 2);
-        else if (rk == 1) JAC_LAUNCH(kJacHNarrow, 1);
-        else JAC_LAUNCH(kJacHNarrow, 0);
-    } else {
-        if (rk == 2) JAC_LAUNCH(kJacH, 2);
-        else if (rk == 1) JAC_LAUNCH(kJacH, 1);
-        else JAC_LAUNCH(kJacH, 0);
+    switch (halo) {
+    case kJacHNarrow: JAC_HALO(kJacHNarrow); break;
+    case kJacH: JAC_HALO(kJacH); break;
+    case 20: JAC_HALO(20); break;
+    case 24: JAC_HALO(24); break;
+    case 28: JAC_HALO(28); break;
+    default: break;  // no other halo is chosen
     }
+#undef JAC_HALO
 #undef JAC_LAUNCH
 }
 void launch_ir_update(const Launch &L, int par) { LAUNCH_K(k_ir_update, L, par); }

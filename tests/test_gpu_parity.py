@@ -135,14 +135,37 @@ def test_charge_conservation_exact(dev):
     assert int(q.sum()) == (ni - ne) * 2**32
 
 
-@pytest.mark.parametrize("grid", [(64, 32, 10), (4096, 257, 1)], ids=["64x32", "4096x257"])
+# One grid per blocked-Jacobi halo a single run takes: the widest halo whose
+# launch has at most 256 workgroups (28, 24 or 20 nodes), else 16, or 10
+# (narrow) when the 16-node halo would need 1024 workgroups or more.
+JAC_GRIDS = [(64, 32, 10), (256, 128, 1), (384, 192, 1), (640, 320, 1), (4096, 257, 1)]
+JAC_HALOS = [28, 24, 20, 16, 10]
+JAC_IDS = [f"{a}x{b}-halo{h}" for (a, b, _), h in zip(JAC_GRIDS, JAC_HALOS)]
+
+
+def jac_halo(ncx, ncy):
+    """The halo the library picks for a single run (restated for the test)."""
+    def blocks(h):
+        t = 64 - 2 * h
+        return -(-ncx // t) * -(-(ncy - 1) // t)
+    for h in (28, 24, 20):
+        if blocks(h) <= 256:
+            return h
+    return 10 if blocks(16) >= 1024 else 16
+
+
+def test_jacobi_grids_cover_every_halo():
+    assert [jac_halo(a, b) for a, b, _ in JAC_GRIDS] == JAC_HALOS
+
+
+@pytest.mark.parametrize("grid", JAC_GRIDS, ids=JAC_IDS)
 @pytest.mark.parametrize("mode", [0, 1, 2])
 def test_jacobi_matches_fp64_sweeps(dev, mode, grid):
     """Same start potential, same charge, same number of sweeps: the GPU
     potential agrees with the FP64 oracle (FP32 sweeps: 2e-6 of max|phi|;
     refinement, whose corrections are FP32 sweeps: 3e-8 after 300 unconverged
-    sweeps).  The 4096 x 257 grid needs 1024 wide-halo workgroups, so the
-    blocked sweeps run with the narrow halo."""
+    sweeps), with every halo of the blocked sweeps (27, 23, 19, 15 and 9
+    sweeps per launch)."""
     ncx, ncy, ppc = grid
     p = small(ncx=ncx, ncy=ncy, ppc=ppc, jacobi_mode=mode, jacobi_max_sweeps=300,
               jacobi_check_every=50)
@@ -186,14 +209,14 @@ def test_jacobi_matches_fp64_sweeps(dev, mode, grid):
     assert H.rel_err(Ey, fy.reshape(Ey.shape)) < 1e-6
 
 
-@pytest.mark.parametrize("grid", [(64, 32, 10), (4096, 257, 1)], ids=["64x32", "4096x257"])
+@pytest.mark.parametrize("grid", JAC_GRIDS, ids=JAC_IDS)
 def test_jacobi_stops_at_the_oracle_check(dev, grid):
     """Mode 1 stopping at a residual check: with the tolerance between the
     FP64 residuals at sweeps 150 and 200, the GPU solve stops at sweep 200 as
     the FP64 oracle does and returns the potential of that sweep.  With the
-    wide halo (64 x 32) the check falls inside a launch of 15 sweeps and the
-    potential comes from the third buffer; with the narrow halo (4096 x 257)
-    launches end at every check."""
+    wide halos the check falls inside a launch (of 27, 23, 19 or 15 sweeps)
+    and the potential comes from the third buffer; with the narrow halo
+    (4096 x 257) launches end at every check."""
     import ctypes as C
     ncx, ncy, ppc = grid
     p = small(ncx=ncx, ncy=ncy, ppc=ppc, jacobi_mode=1, jacobi_max_sweeps=300,
