@@ -365,10 +365,13 @@ void seq_solve(const Target &T, int use_args, int mode, double tol, int min_swee
 
 namespace cupic {
 
-void seq_solve_one(const RunDev *d_rd, const int *d_plan, const int *nblk, hipStream_t stream,
-                   int mode, double tol, int min_sweeps, int max_sweeps, int check_every)
+void seq_solve_one(const RunDev *d_rd, const int *d_plan, const int *nblk, int ncx, int ncy,
+                   hipStream_t stream, int mode, double tol, int min_sweeps, int max_sweeps,
+                   int check_every)
 {
     Target T{d_rd, d_plan, nblk, 1, stream, nullptr};
+    T.jac_h = jac_solo_halo(ncx, ncy);
+    T.jac_nb = jac_blocks(ncx, ncy, T.jac_h);
     seq_solve(T, 1, mode, tol, min_sweeps, max_sweeps, check_every);
 }
 

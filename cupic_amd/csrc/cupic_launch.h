@@ -105,11 +105,12 @@ void launch_far_scatter(const Launch &L, int advance_t);
 void launch_init_finish(const Launch &L);
 
 // host sequences and state shared between translation units
-// The Poisson solve (any mode) of the one run described by d_rd, on stream,
-// with the blocks per work shape in nblk and their prefixes in d_plan
-// (cupic_api.cpp).
-void seq_solve_one(const RunDev *d_rd, const int *d_plan, const int *nblk, hipStream_t stream,
-                   int mode, double tol, int min_sweeps, int max_sweeps, int check_every);
+// The Poisson solve (any mode) of the one run described by d_rd, an ncx x
+// ncy grid, on stream, with the blocks per work shape in nblk and their
+// prefixes in d_plan (cupic_api.cpp).
+void seq_solve_one(const RunDev *d_rd, const int *d_plan, const int *nblk, int ncx, int ncy,
+                   hipStream_t stream, int mode, double tol, int min_sweeps, int max_sweeps,
+                   int check_every);
 // Records msg as the last error of calls without a run (cupic_last_error(NULL));
 // returns code (cupic_api.cpp).
 int set_global_error(int code, const char *msg);

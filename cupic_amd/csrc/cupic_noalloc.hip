@@ -180,8 +180,9 @@ extern "C" int cupic_poisson_solver_noalloc(const cupic_params *pin, const int64
     const int nb = nblk[kShapeNodes];
     hipLaunchKernelGGL(k_noalloc_stage, dim3(nb), dim3(kBlock), 0, stream, d_rd, d_plan, 1,
                        (const long long *)charge);
-    seq_solve_one(d_rd, d_plan, nblk, stream, p.jacobi_mode, tol < 0 ? p.jacobi_tol : tol,
-                  p.jacobi_min_sweeps, p.jacobi_max_sweeps, p.jacobi_check_every);
+    seq_solve_one(d_rd, d_plan, nblk, R.ncx, R.ncy, stream, p.jacobi_mode,
+                  tol < 0 ? p.jacobi_tol : tol, p.jacobi_min_sweeps, p.jacobi_max_sweeps,
+                  p.jacobi_check_every);
     hipLaunchKernelGGL(k_noalloc_publish, dim3(nb), dim3(kBlock), 0, stream, d_rd, d_plan, 1);
     e = hipGetLastError();
     RunState out;
