@@ -166,9 +166,24 @@ def test_jacobi_matches_fp64_sweeps(dev, mode, grid):
     refinement, whose corrections are FP32 sweeps: 3e-8 after 300 unconverged
     sweeps), with every halo of the blocked sweeps (27, 23, 19, 15 and 9
     sweeps per launch)."""
+    jacobi_vs_oracle(mode, grid, 50)
+
+
+@pytest.mark.parametrize("check", [20, 7])
+@pytest.mark.parametrize("grid", [JAC_GRIDS[0], JAC_GRIDS[3]], ids=[JAC_IDS[0], JAC_IDS[3]])
+@pytest.mark.parametrize("mode", [1, 2])
+def test_jacobi_short_check_interval(dev, mode, grid, check):
+    """Residual checks closer together than a launch's sweeps (27 with the
+    28-node halo, 15 with the 16-node one): launches then end at every
+    check, and the potential after 300 sweeps still agrees with the FP64
+    oracle checking at the same sweeps."""
+    jacobi_vs_oracle(mode, grid, check)
+
+
+def jacobi_vs_oracle(mode, grid, check):
     ncx, ncy, ppc = grid
     p = small(ncx=ncx, ncy=ncy, ppc=ppc, jacobi_mode=mode, jacobi_max_sweeps=300,
-              jacobi_check_every=50)
+              jacobi_check_every=check)
     run = cp.Run(p)
     run.init_sim()
     run.step(p.dt)
@@ -191,10 +206,10 @@ def test_jacobi_matches_fp64_sweeps(dev, mode, grid):
     import ctypes as C
     r = C.c_double()
     s2 = O.lib().orc_poisson_solve(p.ncx, p.ncy, p.ds, p.eps0, rho.ravel().copy(), want, mode,
-                                   0.0, 0, 300, 50, C.byref(r))
+                                   0.0, 0, 300, check, C.byref(r))
     assert s2 == 300
     err = H.rel_err(got, want.reshape(got.shape))
-    say("mode", mode, "phi rel err", err, "resid gpu", res, "oracle", r.value)
+    say("mode", mode, "check", check, "phi rel err", err, "resid gpu", res, "oracle", r.value)
     assert err < (3e-8 if mode == 2 else 2e-6)
     if mode:
         assert res == pytest.approx(r.value, rel=1e-3 if mode == 2 else 0.2, abs=2e-7)
